@@ -1,0 +1,538 @@
+// C ABI of libfx_lbm.so (declared in include/fx_lbm.h): device discovery, contexts,
+// buffers, typed kernel launches and FP64 reductions.  Replaces the OpenCL-Wrapper
+// device/buffer/kernel layer of the reference (src/opencl.hpp:284-692).
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <cmath>
+#include <string>
+#include <mutex>
+#include <set>
+#include "fx_lbm.h"
+#include "lbm_launch.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& message) {
+	g_last_error = message;
+	return code;
+}
+int fail_hip(hipError_t e, const char* what) {
+	return fail(FX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define FX_HIP(call) do { const hipError_t fx_e_ = (call); if(fx_e_ != hipSuccess) return fail_hip(fx_e_, #call); } while(0)
+
+bool valid_enums(const fx_lbm_config* c) {
+	return c && (c->velocity_set == FX_D3Q19 || c->velocity_set == FX_D3Q27) && c->collision <= FX_TRT && c->storage <= FX_FP16C &&
+	       (c->extensions & ~(FX_VOLUME_FORCE | FX_FORCE_FIELD | FX_EQUILIBRIUM_BOUNDARIES | FX_UPDATE_FIELDS | FX_POROSITY)) == 0u;
+}
+
+} // namespace
+
+struct fx_ctx {
+	int device = 0;
+	fx_lbm_config cfg{};
+	uint64_t N = 0;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	const fx::LbmKernels* k = nullptr;
+	fx::LaunchShape shape{};
+	double* partial = nullptr; // reduction partials (device)
+	uint64_t partial_cap = 0;  // in doubles
+	double* result = nullptr;  // 4 doubles (device)
+	std::set<void*> buffers;   // allocations owned by this context
+};
+
+namespace {
+
+int set_device(const fx_ctx* ctx) {
+	FX_HIP(hipSetDevice(ctx->device));
+	return FX_OK;
+}
+
+int ensure_partial(fx_ctx* ctx, uint64_t doubles) {
+	if(ctx->partial_cap >= doubles) return FX_OK;
+	if(ctx->partial) FX_HIP(hipFree(ctx->partial));
+	ctx->partial = nullptr;
+	ctx->partial_cap = 0;
+	FX_HIP(hipMalloc(&ctx->partial, doubles * sizeof(double)));
+	ctx->partial_cap = doubles;
+	return FX_OK;
+}
+
+fx::LbmArgs make_args(const fx_ctx* ctx, const void* fi, const float* rho, const float* u, const uint8_t* flags, uint64_t t,
+                      float fx_, float fy_, float fz_, const float* F, const float* alpha) {
+	fx::LbmArgs a;
+	a.fi = const_cast<void*>(fi);
+	a.rho = const_cast<float*>(rho);
+	a.u = const_cast<float*>(u);
+	a.flags = flags;
+	a.F = F;
+	a.alpha = alpha;
+	a.N = ctx->N;
+	a.Nx = ctx->cfg.Nx; a.Ny = ctx->cfg.Ny; a.Nz = ctx->cfg.Nz;
+	a.ext = ctx->cfg.extensions;
+	a.odd = (uint32_t)(t & 1ull);
+	a.w = ctx->cfg.w;
+	a.fx = fx_; a.fy = fy_; a.fz = fz_;
+	return a;
+}
+
+int check_step_args(const fx_ctx* ctx, const void* fi, const float* rho, const float* u, const uint8_t* flags, const float* F, const float* alpha) {
+	if(!ctx) return fail(FX_ERR_ARG, "null context");
+	if(!fi || !flags) return fail(FX_ERR_ARG, "fi and flags must be device pointers");
+	const uint32_t ext = ctx->cfg.extensions;
+	if((ext & FX_FORCE_FIELD) && !F) return fail(FX_ERR_ARG, "FORCE_FIELD is enabled but F is null");
+	if((ext & FX_POROSITY) && !alpha) return fail(FX_ERR_ARG, "POROSITY is enabled but alpha is null");
+	if((ext & (FX_EQUILIBRIUM_BOUNDARIES | FX_UPDATE_FIELDS)) && (!rho || !u)) return fail(FX_ERR_ARG, "rho and u are required with EQUILIBRIUM_BOUNDARIES or UPDATE_FIELDS");
+	return FX_OK;
+}
+
+int check_launch(const char* what) {
+	const hipError_t e = hipGetLastError();
+	if(e != hipSuccess) return fail_hip(e, what);
+	return FX_OK;
+}
+
+// Relaxation-rate text exactly as the reference prints it: split_float scales x into
+// [1,10) with float multiplications, keeps 8 decimals of the fraction (rounded half up
+// in float arithmetic) and prints "I.DDDDDDDD[E<exp>]"
+// (src/utilities.hpp:2599-2621, 2669-2675, 2745-2754).
+std::string reference_float_text(float x) {
+	std::string text = x < 0.0f ? "-" : "";
+	x = std::fabs(x);
+	if(std::isnan(x)) return text + "NaN";
+	if(std::isinf(x)) return text + "Inf";
+	int e10 = 0;
+	static const float up[6] = {1E32f, 1E16f, 1E8f, 1E4f, 1E2f, 1E1f};
+	static const float down[6] = {1E-32f, 1E-16f, 1E-8f, 1E-4f, 1E-2f, 1E-1f};
+	static const float small[6] = {1E-31f, 1E-15f, 1E-7f, 1E-3f, 1E-1f, 1E0f};
+	static const int step[6] = {32, 16, 8, 4, 2, 1};
+	if(x >= 10.0f) {
+		for(int k = 0; k < 6; k++) if(x >= up[k]) { x *= down[k]; e10 += step[k]; }
+	}
+	if(x > 0.0f && x <= 1.0f) {
+		for(int k = 0; k < 6; k++) if(x < small[k]) { x *= up[k]; e10 -= step[k]; }
+	}
+	unsigned int whole = (unsigned int)x;
+	const float frac8 = (x - (float)whole) * 1E8f;
+	unsigned int digits = (unsigned int)frac8;
+	if(frac8 - (float)digits >= 0.5f && ++digits >= 100000000u) {
+		digits = 0u;
+		if(++whole >= 10u) { whole = 1u; e10++; }
+	}
+	char buf[48];
+	std::snprintf(buf, sizeof(buf), "%u.%08u", whole, digits);
+	text += buf;
+	if(e10 != 0) text += "E" + std::to_string(e10);
+	return text;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------------------
+// Device kernels that do not depend on the lattice variant
+// ---------------------------------------------------------------------------------------
+namespace fx {
+
+constexpr int RED_BLOCK = 256;
+constexpr int RED_GRID = 1024;
+
+// Object sums (src/kernel.cpp:1901-1959): mode 0 force, 1 torque about (cx,cy,cz),
+// 2 centre of mass (positions and cell count).  Per-cell terms in FP32 as the reference
+// computes them, accumulated in FP64; one partial (4 doubles) per block.
+__global__ __launch_bounds__(RED_BLOCK) void k_object_partial(const float* F, const uint8_t* flags, uint32_t marker, uint64_t N,
+                                                              uint32_t Nx, uint32_t Ny, uint32_t Nz, int mode, float cx, float cy, float cz,
+                                                              double* partial) {
+	__shared__ double red[4][RED_BLOCK];
+	double acc[4] = {0.0, 0.0, 0.0, 0.0};
+	const uint64_t NxNy = (uint64_t)Nx * Ny;
+	for(uint64_t n = (uint64_t)blockIdx.x * RED_BLOCK + threadIdx.x; n < N; n += (uint64_t)gridDim.x * RED_BLOCK) {
+		if(flags[n] != marker) continue;
+		if(mode == 0) {
+			acc[0] += (double)F[n]; acc[1] += (double)F[N + n]; acc[2] += (double)F[2ull * N + n];
+		} else {
+			const uint64_t tt = n % NxNy;
+			const uint32_t x = (uint32_t)(tt % Nx), y = (uint32_t)(tt / Nx), z = (uint32_t)(n / NxNy);
+			const float px = (float)x + 0.5f - 0.5f * (float)Nx; // position(), src/kernel.cpp:824-826
+			const float py = (float)y + 0.5f - 0.5f * (float)Ny;
+			const float pz = (float)z + 0.5f - 0.5f * (float)Nz;
+			if(mode == 1) {
+				const float rx = px - cx, ry = py - cy, rz = pz - cz;
+				const float Fx = F[n], Fy = F[N + n], Fz = F[2ull * N + n];
+				acc[0] += (double)(ry * Fz - rz * Fy);
+				acc[1] += (double)(rz * Fx - rx * Fz);
+				acc[2] += (double)(rx * Fy - ry * Fx);
+			} else {
+				acc[0] += (double)px; acc[1] += (double)py; acc[2] += (double)pz; acc[3] += 1.0;
+			}
+		}
+	}
+	for(int k = 0; k < 4; k++) red[k][threadIdx.x] = acc[k];
+	__syncthreads();
+	for(int s = RED_BLOCK / 2; s > 0; s >>= 1) {
+		if((int)threadIdx.x < s) for(int k = 0; k < 4; k++) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+		__syncthreads();
+	}
+	if(threadIdx.x < 4) partial[(uint64_t)blockIdx.x * 4u + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Sum `count` partials of `stride` doubles each, component by component, in a fixed
+// order (single block): out[k] = sum_b partial[b*stride+k].
+__global__ __launch_bounds__(RED_BLOCK) void k_reduce_partials(const double* partial, uint64_t count, uint32_t stride, double* out) {
+	__shared__ double red[RED_BLOCK];
+	for(uint32_t k = 0; k < stride; k++) {
+		double s = 0.0;
+		for(uint64_t b = threadIdx.x; b < count; b += RED_BLOCK) s += partial[b * stride + k];
+		red[threadIdx.x] = s;
+		__syncthreads();
+		for(int h = RED_BLOCK / 2; h > 0; h >>= 1) {
+			if((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+			__syncthreads();
+		}
+		if(threadIdx.x == 0) out[k] = red[0];
+		__syncthreads();
+	}
+}
+
+__global__ void k_reset3(float* F, uint64_t n3) {
+	const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if(i < n3) F[i] = 0.0f;
+}
+
+template<uint32_t ST> __global__ void k_codec(const float* x, uint16_t* codes, float* decoded, uint64_t n) {
+	const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= n) return;
+	const uint16_t c = Storage<ST>::encode(x[i]);
+	codes[i] = c;
+	decoded[i] = Storage<ST>::decode(c);
+}
+
+__global__ __launch_bounds__(256) void k_copy16(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n16) {
+	const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+	if(i < n16) dst[i] = src[i];
+}
+
+} // namespace fx
+
+// ---------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+const char* fx_last_error(void) { return g_last_error.c_str(); }
+
+int fx_device_count(int* count) {
+	if(!count) return fail(FX_ERR_ARG, "null count");
+	int c = 0;
+	const hipError_t e = hipGetDeviceCount(&c);
+	if(e == hipErrorNoDevice) { *count = 0; return FX_OK; }
+	if(e != hipSuccess) return fail_hip(e, "hipGetDeviceCount");
+	*count = c;
+	return FX_OK;
+}
+
+int fx_device_info_get(int device, fx_device_info* out) {
+	if(!out) return fail(FX_ERR_ARG, "null output");
+	hipDeviceProp_t p;
+	FX_HIP(hipGetDeviceProperties(&p, device));
+	std::memset(out, 0, sizeof(*out));
+	std::snprintf(out->name, sizeof(out->name), "%s", p.name);
+	std::snprintf(out->arch, sizeof(out->arch), "%s", p.gcnArchName);
+	out->memory_bytes = p.totalGlobalMem;
+	out->compute_units = (uint32_t)p.multiProcessorCount;
+	out->clock_mhz = (uint32_t)(p.clockRate / 1000);
+	int current = 0;
+	FX_HIP(hipGetDevice(&current));
+	FX_HIP(hipSetDevice(device));
+	size_t free_b = 0, total_b = 0;
+	const hipError_t e = hipMemGetInfo(&free_b, &total_b);
+	(void)hipSetDevice(current);
+	if(e != hipSuccess) return fail_hip(e, "hipMemGetInfo");
+	out->memory_free = free_b;
+	return FX_OK;
+}
+
+float fx_relaxation_rate(float nu) {
+	const float tau = 3.0f * nu + 0.5f;
+	return std::strtof(reference_float_text(1.0f / tau).c_str(), nullptr);
+}
+
+uint64_t fx_memory_bytes_per_cell(const fx_lbm_config* c) {
+	if(!valid_enums(c)) return 0;
+	uint64_t b = (uint64_t)c->velocity_set * (c->storage == FX_FP32 ? 4u : 2u) + 17u; // fi, rho, u, flags
+	if(c->extensions & FX_FORCE_FIELD) b += 12u;
+	if(c->extensions & FX_POROSITY) b += 4u;
+	return b;
+}
+
+uint64_t fx_bandwidth_bytes_per_cell(const fx_lbm_config* c) {
+	if(!valid_enums(c)) return 0;
+	uint64_t b = (uint64_t)c->velocity_set * 2u * (c->storage == FX_FP32 ? 4u : 2u) + 1u; // q loads + q stores + flags
+	if(c->extensions & FX_UPDATE_FIELDS) b += 16u;
+	if(c->extensions & FX_FORCE_FIELD) b += 12u;
+	if(c->extensions & FX_POROSITY) b += 4u;
+	return b;
+}
+
+int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
+	if(!out) return fail(FX_ERR_ARG, "null output handle");
+	*out = nullptr;
+	if(!cfg) return fail(FX_ERR_ARG, "null config");
+	if(!valid_enums(cfg)) return fail(FX_ERR_CONFIG, "unsupported velocity set / collision / storage / extension bits");
+	const uint64_t N = (uint64_t)cfg->Nx * cfg->Ny * cfg->Nz;
+	if(N == 0ull) return fail(FX_ERR_CONFIG, "Grid point number is 0");
+	if(cfg->Ny > 65536u || cfg->Nz > 65536u) return fail(FX_ERR_CONFIG, "Ny and Nz must be at most 65536");
+	if(!(cfg->w > 0.0f && cfg->w < 2.0f)) return fail(FX_ERR_CONFIG, "relaxation rate w must be in (0,2): viscosity must be positive");
+	int count = 0;
+	const int rc = fx_device_count(&count);
+	if(rc != FX_OK) return rc;
+	if(device < 0 || device >= count) return fail(FX_ERR_ARG, "no such device");
+	fx_ctx* ctx = new fx_ctx();
+	ctx->device = device;
+	ctx->cfg = *cfg;
+	ctx->N = N;
+	ctx->k = cfg->velocity_set == FX_D3Q19 ? fx::kernels_q19(cfg->storage, cfg->collision == FX_TRT)
+	                                        : fx::kernels_q27(cfg->storage, cfg->collision == FX_TRT);
+	ctx->shape = fx::lattice_shape(cfg->Nx, cfg->Ny, cfg->Nz);
+	hipError_t e = hipSetDevice(device);
+	if(e == hipSuccess) {
+		size_t free_b = 0, total_b = 0;
+		e = hipMemGetInfo(&free_b, &total_b);
+		if(e == hipSuccess && N * fx_memory_bytes_per_cell(cfg) > (uint64_t)free_b) {
+			delete ctx;
+			return fail(FX_ERR_MEMORY, "Grid resolution is too large for the free device memory");
+		}
+	}
+	if(e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+	if(e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+	if(e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+	if(e == hipSuccess) e = hipMalloc(&ctx->result, 4 * sizeof(double));
+	if(e != hipSuccess) {
+		fx_ctx_destroy(ctx);
+		return fail_hip(e, "fx_ctx_create");
+	}
+	*out = ctx;
+	return FX_OK;
+}
+
+int fx_ctx_destroy(fx_ctx* ctx) {
+	if(!ctx) return FX_OK;
+	(void)hipSetDevice(ctx->device);
+	if(ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+	for(void* p : ctx->buffers) (void)hipFree(p);
+	if(ctx->partial) (void)hipFree(ctx->partial);
+	if(ctx->result) (void)hipFree(ctx->result);
+	if(ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+	if(ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+	if(ctx->stream) (void)hipStreamDestroy(ctx->stream);
+	delete ctx;
+	return FX_OK;
+}
+
+int fx_ctx_config(const fx_ctx* ctx, fx_lbm_config* out) {
+	if(!ctx || !out) return fail(FX_ERR_ARG, "null argument");
+	*out = ctx->cfg;
+	return FX_OK;
+}
+
+int fx_sync(fx_ctx* ctx) {
+	if(!ctx) return fail(FX_ERR_ARG, "null context");
+	FX_HIP(hipStreamSynchronize(ctx->stream));
+	return FX_OK;
+}
+
+int fx_buf_alloc(fx_ctx* ctx, uint64_t bytes, void** dptr) {
+	if(!ctx || !dptr) return fail(FX_ERR_ARG, "null argument");
+	*dptr = nullptr;
+	if(bytes == 0ull) return fail(FX_ERR_ARG, "Memory size must be larger than 0");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	void* p = nullptr;
+	const hipError_t e = hipMalloc(&p, bytes);
+	if(e != hipSuccess) return fail(FX_ERR_MEMORY, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+	ctx->buffers.insert(p);
+	*dptr = p;
+	return FX_OK;
+}
+
+int fx_buf_free(fx_ctx* ctx, void* dptr) {
+	if(!ctx) return fail(FX_ERR_ARG, "null context");
+	if(!dptr) return FX_OK;
+	auto it = ctx->buffers.find(dptr);
+	if(it == ctx->buffers.end()) return fail(FX_ERR_ARG, "pointer was not allocated by this context");
+	FX_HIP(hipStreamSynchronize(ctx->stream));
+	FX_HIP(hipFree(dptr));
+	ctx->buffers.erase(it);
+	return FX_OK;
+}
+
+int fx_buf_write(fx_ctx* ctx, void* dst, uint64_t offset, uint64_t bytes, const void* host, int blocking) {
+	if(!ctx || !dst || (!host && bytes)) return fail(FX_ERR_ARG, "null argument");
+	FX_HIP(hipMemcpyAsync((char*)dst + offset, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+	if(blocking) FX_HIP(hipStreamSynchronize(ctx->stream));
+	return FX_OK;
+}
+
+int fx_buf_read(fx_ctx* ctx, const void* src, uint64_t offset, uint64_t bytes, void* host, int blocking) {
+	if(!ctx || !src || (!host && bytes)) return fail(FX_ERR_ARG, "null argument");
+	FX_HIP(hipMemcpyAsync(host, (const char*)src + offset, bytes, hipMemcpyDeviceToHost, ctx->stream));
+	if(blocking) FX_HIP(hipStreamSynchronize(ctx->stream));
+	return FX_OK;
+}
+
+int fx_buf_fill(fx_ctx* ctx, void* dst, uint64_t bytes, uint8_t value) {
+	if(!ctx || !dst) return fail(FX_ERR_ARG, "null argument");
+	FX_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+	return FX_OK;
+}
+
+int fx_buf_copy(fx_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+	if(!ctx || !dst || !src) return fail(FX_ERR_ARG, "null argument");
+	FX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+	return FX_OK;
+}
+
+int fx_lbm_initialize(fx_ctx* ctx, void* fi, const float* rho, float* u, const uint8_t* flags) {
+	if(!ctx) return fail(FX_ERR_ARG, "null context");
+	if(!fi || !rho || !u || !flags) return fail(FX_ERR_ARG, "fi, rho, u and flags must be device pointers");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	const fx::LbmArgs a = make_args(ctx, fi, rho, u, flags, 1ull, 0.0f, 0.0f, 0.0f, nullptr, nullptr);
+	ctx->k->initialize(a, ctx->shape, ctx->stream);
+	return check_launch("initialize");
+}
+
+int fx_lbm_stream_collide(fx_ctx* ctx, void* fi, float* rho, float* u, const uint8_t* flags, uint64_t t, float fx_, float fy_, float fz_,
+                          const float* F, const float* alpha) {
+	const int rc = check_step_args(ctx, fi, rho, u, flags, F, alpha);
+	if(rc != FX_OK) return rc;
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	const fx::LbmArgs a = make_args(ctx, fi, rho, u, flags, t, fx_, fy_, fz_, F, alpha);
+	ctx->k->stream_collide(a, ctx->shape, ctx->stream);
+	return check_launch("stream_collide");
+}
+
+int fx_lbm_run(fx_ctx* ctx, void* fi, float* rho, float* u, const uint8_t* flags, uint64_t t0, uint64_t steps, float fx_, float fy_, float fz_,
+               const float* F, const float* alpha, float* ms) {
+	const int rc = check_step_args(ctx, fi, rho, u, flags, F, alpha);
+	if(rc != FX_OK) return rc;
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	FX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+	fx::LbmArgs a = make_args(ctx, fi, rho, u, flags, t0, fx_, fy_, fz_, F, alpha);
+	for(uint64_t s = 0; s < steps; s++) {
+		a.odd = (uint32_t)((t0 + s) & 1ull);
+		ctx->k->stream_collide(a, ctx->shape, ctx->stream);
+	}
+	const int lc = check_launch("stream_collide");
+	if(lc != FX_OK) return lc;
+	FX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+	if(ms) {
+		FX_HIP(hipEventSynchronize(ctx->ev1));
+		FX_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+	}
+	return FX_OK;
+}
+
+int fx_lbm_update_fields(fx_ctx* ctx, const void* fi, float* rho, float* u, const uint8_t* flags, uint64_t t, float fx_, float fy_, float fz_,
+                         const float* F, const float* alpha) {
+	const int rc = check_step_args(ctx, fi, rho, u, flags, F, alpha);
+	if(rc != FX_OK) return rc;
+	if(!rho || !u) return fail(FX_ERR_ARG, "rho and u must be device pointers");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	const fx::LbmArgs a = make_args(ctx, fi, rho, u, flags, t, fx_, fy_, fz_, F, alpha);
+	ctx->k->update_fields(a, ctx->shape, ctx->stream);
+	return check_launch("update_fields");
+}
+
+int fx_lbm_update_force_field(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t t, float* F) {
+	if(!ctx) return fail(FX_ERR_ARG, "null context");
+	if(!fi || !flags || !F) return fail(FX_ERR_ARG, "fi, flags and F must be device pointers");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	const fx::LbmArgs a = make_args(ctx, fi, nullptr, nullptr, flags, t, 0.0f, 0.0f, 0.0f, F, nullptr);
+	ctx->k->update_force_field(a, ctx->shape, ctx->stream);
+	return check_launch("update_force_field");
+}
+
+int fx_lbm_reset_force_field(fx_ctx* ctx, float* F) {
+	if(!ctx || !F) return fail(FX_ERR_ARG, "null argument");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	const uint64_t n3 = 3ull * ctx->N;
+	fx::k_reset3<<<(unsigned)((n3 + 255u) / 256u), 256, 0, ctx->stream>>>(F, n3);
+	return check_launch("reset_force_field");
+}
+
+static int object_sum(fx_ctx* ctx, const float* F, const uint8_t* flags, uint8_t marker, int mode, float cx, float cy, float cz, double* out, int count) {
+	if(!ctx || !flags || !out || (mode != 2 && !F)) return fail(FX_ERR_ARG, "null argument");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	int rc = ensure_partial(ctx, (uint64_t)fx::RED_GRID * 4u);
+	if(rc != FX_OK) return rc;
+	fx::k_object_partial<<<fx::RED_GRID, fx::RED_BLOCK, 0, ctx->stream>>>(F, flags, marker, ctx->N, ctx->cfg.Nx, ctx->cfg.Ny, ctx->cfg.Nz, mode, cx, cy, cz, ctx->partial);
+	fx::k_reduce_partials<<<1, fx::RED_BLOCK, 0, ctx->stream>>>(ctx->partial, fx::RED_GRID, 4u, ctx->result);
+	rc = check_launch("object sum");
+	if(rc != FX_OK) return rc;
+	double h[4];
+	FX_HIP(hipMemcpyAsync(h, ctx->result, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+	FX_HIP(hipStreamSynchronize(ctx->stream));
+	for(int k = 0; k < count; k++) out[k] = h[k];
+	return FX_OK;
+}
+
+int fx_lbm_object_force(fx_ctx* ctx, const float* F, const uint8_t* flags, uint8_t marker, double out[3]) {
+	return object_sum(ctx, F, flags, marker, 0, 0.0f, 0.0f, 0.0f, out, 3);
+}
+int fx_lbm_object_torque(fx_ctx* ctx, const float* F, const uint8_t* flags, uint8_t marker, float cx, float cy, float cz, double out[3]) {
+	return object_sum(ctx, F, flags, marker, 1, cx, cy, cz, out, 3);
+}
+int fx_lbm_object_center_of_mass(fx_ctx* ctx, const uint8_t* flags, uint8_t marker, double out[4]) {
+	return object_sum(ctx, nullptr, flags, marker, 2, 0.0f, 0.0f, 0.0f, out, 4);
+}
+
+int fx_lbm_mass_sum(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t t, double* out) {
+	if(!ctx || !fi || !flags || !out) return fail(FX_ERR_ARG, "null argument");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	const uint64_t blocks = (uint64_t)ctx->shape.grid.x * ctx->shape.grid.y * ctx->shape.grid.z;
+	int rc = ensure_partial(ctx, blocks);
+	if(rc != FX_OK) return rc;
+	const fx::LbmArgs a = make_args(ctx, fi, nullptr, nullptr, flags, t, 0.0f, 0.0f, 0.0f, nullptr, nullptr);
+	ctx->k->mass_partial(a, ctx->shape, ctx->stream, ctx->partial);
+	fx::k_reduce_partials<<<1, fx::RED_BLOCK, 0, ctx->stream>>>(ctx->partial, blocks, 1u, ctx->result);
+	rc = check_launch("mass sum");
+	if(rc != FX_OK) return rc;
+	FX_HIP(hipMemcpyAsync(out, ctx->result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+	FX_HIP(hipStreamSynchronize(ctx->stream));
+	return FX_OK;
+}
+
+int fx_codec_roundtrip(fx_ctx* ctx, uint32_t storage, const float* x, uint16_t* codes, float* decoded, uint64_t n) {
+	if(!ctx || !x || !codes || !decoded) return fail(FX_ERR_ARG, "null argument");
+	if(storage != FX_FP16S && storage != FX_FP16C) return fail(FX_ERR_ARG, "storage must be FX_FP16S or FX_FP16C");
+	if(n == 0) return FX_OK;
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	const unsigned blocks = (unsigned)((n + 255u) / 256u);
+	if(storage == FX_FP16S) fx::k_codec<fx::ST_FP16S><<<blocks, 256, 0, ctx->stream>>>(x, codes, decoded, n);
+	else fx::k_codec<fx::ST_FP16C><<<blocks, 256, 0, ctx->stream>>>(x, codes, decoded, n);
+	return check_launch("codec");
+}
+
+int fx_bench_copy(fx_ctx* ctx, void* dst, const void* src, uint64_t bytes, uint32_t reps, float* ms) {
+	if(!ctx || !dst || !src) return fail(FX_ERR_ARG, "null argument");
+	if(bytes % 16u != 0u || bytes == 0u) return fail(FX_ERR_ARG, "bytes must be a positive multiple of 16");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	const uint64_t n16 = bytes / 16u;
+	const uint64_t blocks = (n16 + 255u) / 256u;
+	if(blocks > 0x7FFFFFFFull) return fail(FX_ERR_ARG, "copy too large");
+	FX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+	for(uint32_t r = 0; r < reps; r++) fx::k_copy16<<<(unsigned)blocks, 256, 0, ctx->stream>>>((const float4*)src, (float4*)dst, n16);
+	const int rc = check_launch("copy");
+	if(rc != FX_OK) return rc;
+	FX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+	FX_HIP(hipEventSynchronize(ctx->ev1));
+	float t = 0.0f;
+	FX_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+	if(ms) *ms = t;
+	return FX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,444 @@
+// Device code of the LBM hot path for gfx950: storage codecs, per-cell LBM arithmetic
+// and the Esoteric-Pull in-place access pattern.  Included by the per-velocity-set
+// translation units (lbm_q19.hip, lbm_q27.hip).
+//
+// Arithmetic follows the reference kernels operation by operation (src/kernel.cpp
+// :1004-1101, 1454-1636, 1794-1870): explicit fmaf() where the reference writes fma(),
+// everything else uncontracted (built with -ffp-contract=off), IEEE division.  Terms
+// whose lattice coefficient is 0 are left out, as the reference's compiler does under
+// -cl-no-signed-zeros (src/opencl.hpp:316); results agree with the oracle up to the
+// sign of zero.
+//
+// Thread mapping: one lane per cell.  A block covers BX consecutive cells of one x-row;
+// grid = (ceil(Nx/BX), Ny, Nz).  Everything that depends only on (y, z) - the periodic
+// row offsets of the 9 neighbour rows and the slot base pointers - is wave-uniform and
+// lives in scalar registers; per lane only x, x+1, x-1 (with wrap) remain, so every DDF
+// access is a scalar-base + 32-bit lane offset load/store, coalesced along x.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fx {
+
+enum : uint32_t {
+	EXT_VOLUME_FORCE = 1u, EXT_FORCE_FIELD = 2u, EXT_EQUILIBRIUM_BOUNDARIES = 4u,
+	EXT_UPDATE_FIELDS = 8u, EXT_POROSITY = 16u
+};
+enum : uint32_t { ST_FP32 = 0u, ST_FP16S = 1u, ST_FP16C = 2u };
+constexpr uint32_t TYPE_S = 0x01u, TYPE_E = 0x02u, TYPE_BO = 0x03u, TYPE_SU = 0x38u, TYPE_G = 0x20u;
+constexpr float DEF_C = 0.57735027f; // lattice speed of sound, src/lbm.cpp:366
+constexpr float THIRD = 0.33333334f; // src/kernel.cpp:1093-1099
+
+// Kernel arguments, passed by value (kernarg segment -> scalar registers).
+struct LbmArgs {
+	void* fi;
+	float* rho;
+	float* u;
+	const uint8_t* flags;
+	const float* F;
+	const float* alpha;
+	uint64_t N;
+	uint32_t Nx, Ny, Nz;
+	uint32_t ext;
+	uint32_t odd;   // parity of the time step t
+	float w;
+	float fx, fy, fz;
+};
+
+// ------------------------------------------------------------------------------------
+// DDF storage formats
+// ------------------------------------------------------------------------------------
+template<uint32_t ST> struct Storage;
+template<> struct Storage<ST_FP32> {
+	typedef float E;
+	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return p[o]; }
+	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { p[o] = x; }
+};
+// FP16S (src/lbm.cpp:410-414): IEEE binary16 of x*2^15, round to nearest even
+// (v_cvt_f16_f32 in the default rounding mode), load = binary16 * 2^-15.
+template<> struct Storage<ST_FP16S> {
+	typedef uint16_t E;
+	__device__ static __forceinline__ float decode(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h) * 3.0517578E-5f; }
+	__device__ static __forceinline__ uint16_t encode(float x) { return __builtin_bit_cast(uint16_t, (_Float16)(x * 32768.0f)); }
+	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(p[o]); }
+	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { p[o] = encode(x); }
+};
+// FP16C: 1 sign, 4 exponent (bias 15), 11 mantissa bits (src/kernel.cpp:848-859).
+// Decode without a leading-zero count: a normal code is the top bits of a binary32 with
+// the exponent rebiased by 112; a denormal code is m*2^-25, exact through the integer
+// to float conversion.  Encode is the reference's device rule (add 0x800, truncate; no
+// saturation), written branch-free.
+template<> struct Storage<ST_FP16C> {
+	typedef uint16_t E;
+	__device__ static __forceinline__ float decode(uint16_t h) {
+		const uint32_t a = (uint32_t)h & 0x7FFFu;
+		const float normal = __uint_as_float((a << 12) + (112u << 23));
+		const float denormal = (float)a * 2.98023223876953125E-8f; // a*2^-25
+		const float v = a >= 0x800u ? normal : denormal;
+		return __uint_as_float(__float_as_uint(v) | (((uint32_t)h & 0x8000u) << 16));
+	}
+	__device__ static __forceinline__ uint16_t encode(float x) {
+		const uint32_t b = __float_as_uint(x) + 0x00000800u;
+		const uint32_t e = (b >> 23) & 0xFFu;
+		const uint32_t m = b & 0x007FFFFFu;
+		const uint32_t normal = ((((e - 112u) << 11) & 0x7800u) | (m >> 12));
+		const uint32_t shift = 124u - e; // 12..23 where used
+		const uint32_t denormal = (((0x007FF800u + m) >> (shift & 31u)) + 1u) >> 1;
+		const uint32_t mag = e > 112u ? normal : (e > 100u ? denormal : 0u);
+		return (uint16_t)(((b >> 16) & 0x8000u) | mag);
+	}
+	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(p[o]); }
+	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { p[o] = encode(x); }
+};
+
+// ------------------------------------------------------------------------------------
+// Velocity sets (src/kernel.cpp:864-903; weights src/lbm.cpp:376-385).  The slot order
+// is part of the memory layout, so it is the reference's.
+// ------------------------------------------------------------------------------------
+template<int Q> struct VS;
+template<> struct VS<19> {
+	static constexpr int8_t cx[19] = {0, 1,-1, 0, 0, 0, 0, 1,-1, 1,-1, 0, 0, 1,-1, 1,-1, 0, 0};
+	static constexpr int8_t cy[19] = {0, 0, 0, 1,-1, 0, 0, 1,-1, 0, 0, 1,-1,-1, 1, 0, 0, 1,-1};
+	static constexpr int8_t cz[19] = {0, 0, 0, 0, 0, 1,-1, 0, 0, 1,-1, 1,-1, 0, 0,-1, 1,-1, 1};
+	static constexpr float w0 = 1.0f / 3.0f, ws = 1.0f / 18.0f, we = 1.0f / 36.0f, wc = 0.0f;
+};
+template<> struct VS<27> {
+	static constexpr int8_t cx[27] = {0, 1,-1, 0, 0, 0, 0, 1,-1, 1,-1, 0, 0, 1,-1, 1,-1, 0, 0, 1,-1, 1,-1, 1,-1,-1, 1};
+	static constexpr int8_t cy[27] = {0, 0, 0, 1,-1, 0, 0, 1,-1, 0, 0, 1,-1,-1, 1, 0, 0, 1,-1, 1,-1, 1,-1,-1, 1, 1,-1};
+	static constexpr int8_t cz[27] = {0, 0, 0, 0, 0, 1,-1, 0, 0, 1,-1, 1,-1, 0, 0,-1, 1,-1, 1, 1,-1,-1, 1, 1,-1, 1,-1};
+	static constexpr float w0 = 1.0f / 3.375f, ws = 1.0f / 13.5f, we = 1.0f / 54.0f, wc = 1.0f / 216.0f;
+};
+template<int Q> __device__ __forceinline__ constexpr float weight(int i) {
+	return i == 0 ? VS<Q>::w0 : (VS<Q>::cx[i] * VS<Q>::cx[i] + VS<Q>::cy[i] * VS<Q>::cy[i] + VS<Q>::cz[i] * VS<Q>::cz[i]) == 1 ? VS<Q>::ws
+	     : (VS<Q>::cx[i] * VS<Q>::cx[i] + VS<Q>::cy[i] * VS<Q>::cy[i] + VS<Q>::cz[i] * VS<Q>::cz[i]) == 2 ? VS<Q>::we : VS<Q>::wc;
+}
+
+// ------------------------------------------------------------------------------------
+// Per-cell arithmetic
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ float clampc(float x) { return fminf(fmaxf(x, -DEF_C), DEF_C); } // OpenCL clamp
+
+__device__ __forceinline__ void eq_pair(float* feq, int i, float rw, float rm1w, float uk, float c3) {
+	const float a = __builtin_fmaf(uk, uk, c3);
+	feq[i] = __builtin_fmaf(rw, __builtin_fmaf(0.5f, a, uk), rm1w);
+	feq[i + 1] = __builtin_fmaf(rw, __builtin_fmaf(0.5f, a, -uk), rm1w);
+}
+// calculate_f_eq (src/kernel.cpp:1004-1061), DDF-shifted equilibrium
+template<int Q> __device__ __forceinline__ void calc_feq(float rho, float ux, float uy, float uz, float* feq) {
+	const float rhom1 = rho - 1.0f;
+	const float c3 = -3.0f * ((ux * ux + uy * uy) + uz * uz);
+	uz *= 3.0f; ux *= 3.0f; uy *= 3.0f;
+	feq[0] = VS<Q>::w0 * __builtin_fmaf(rho, 0.5f * c3, rhom1);
+	const float rs = VS<Q>::ws * rho, re = VS<Q>::we * rho, rm1s = VS<Q>::ws * rhom1, rm1e = VS<Q>::we * rhom1;
+	eq_pair(feq, 1, rs, rm1s, ux, c3);
+	eq_pair(feq, 3, rs, rm1s, uy, c3);
+	eq_pair(feq, 5, rs, rm1s, uz, c3);
+	eq_pair(feq, 7, re, rm1e, ux + uy, c3);
+	eq_pair(feq, 9, re, rm1e, ux + uz, c3);
+	eq_pair(feq, 11, re, rm1e, uy + uz, c3);
+	eq_pair(feq, 13, re, rm1e, ux - uy, c3);
+	eq_pair(feq, 15, re, rm1e, ux - uz, c3);
+	eq_pair(feq, 17, re, rm1e, uy - uz, c3);
+	if constexpr(Q == 27) {
+		const float rc = VS<Q>::wc * rho, rm1c = VS<Q>::wc * rhom1;
+		eq_pair(feq, 19, rc, rm1c, (ux + uy) + uz, c3);
+		eq_pair(feq, 21, rc, rm1c, (ux + uy) - uz, c3);
+		eq_pair(feq, 23, rc, rm1c, (ux - uy) + uz, c3);
+		eq_pair(feq, 25, rc, rm1c, (-ux + uy) + uz, c3);
+	}
+}
+
+// calculate_rho_u (src/kernel.cpp:1063-1088): rho summed in slot order, +1 last (DDF
+// shifting); velocity sums in the reference's alternating order.
+template<int Q> __device__ __forceinline__ void calc_rho_u(const float* f, float& rho, float& ux, float& uy, float& uz) {
+	float r = f[0];
+#pragma unroll
+	for(int i = 1; i < Q; i++) r += f[i];
+	r += 1.0f;
+	float jx, jy, jz;
+	if constexpr(Q == 19) {
+		jx = f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[13] - f[14] + f[15] - f[16];
+		jy = f[3] - f[4] + f[7] - f[8] + f[11] - f[12] + f[14] - f[13] + f[17] - f[18];
+		jz = f[5] - f[6] + f[9] - f[10] + f[11] - f[12] + f[16] - f[15] + f[18] - f[17];
+	} else {
+		jx = f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[13] - f[14] + f[15] - f[16] + f[19] - f[20] + f[21] - f[22] + f[23] - f[24] + f[26] - f[25];
+		jy = f[3] - f[4] + f[7] - f[8] + f[11] - f[12] + f[14] - f[13] + f[17] - f[18] + f[19] - f[20] + f[21] - f[22] + f[24] - f[23] + f[25] - f[26];
+		jz = f[5] - f[6] + f[9] - f[10] + f[11] - f[12] + f[16] - f[15] + f[18] - f[17] + f[19] - f[20] + f[22] - f[21] + f[23] - f[24] + f[25] - f[26];
+	}
+	rho = r;
+	ux = jx / r;
+	uy = jy / r;
+	uz = jz / r;
+}
+
+// c_i.v with the zero-coefficient terms left out, summed x, y, z in order
+template<int Q> __device__ __forceinline__ float cdot(int i, float x, float y, float z) {
+	const int a = VS<Q>::cx[i], b = VS<Q>::cy[i], c = VS<Q>::cz[i];
+	float s;
+	bool any = false;
+	s = 0.0f;
+	if(a != 0) { s = a > 0 ? x : -x; any = true; }
+	if(b != 0) { s = any ? (b > 0 ? s + y : s - y) : (b > 0 ? y : -y); any = true; }
+	if(c != 0) { s = any ? (c > 0 ? s + z : s - z) : (c > 0 ? z : -z); }
+	return s;
+}
+// calculate_forcing_terms (src/kernel.cpp:1090-1101), Guo forcing
+template<int Q> __device__ __forceinline__ void calc_forcing(float ux, float uy, float uz, float fx, float fy, float fz, float* Fin) {
+	const float uF = -THIRD * __builtin_fmaf(ux, fx, __builtin_fmaf(uy, fy, uz * fz));
+	Fin[0] = (9.0f * VS<Q>::w0) * uF;
+#pragma unroll
+	for(int i = 1; i < Q; i++) {
+		Fin[i] = (9.0f * weight<Q>(i)) * __builtin_fmaf(cdot<Q>(i, fx, fy, fz), cdot<Q>(i, ux, uy, uz) + THIRD, uF);
+	}
+}
+
+// Force block of stream_collide/update_fields (src/kernel.cpp:1549-1563, 1847-1859)
+// plus the Brinkman porosity term: u = (u + f*0.5/rho)/(1+alpha/2), F = f - alpha*rho*u.
+__device__ __forceinline__ void force_block(uint32_t ext, float rhon, float& ux, float& uy, float& uz,
+                                            float& fxn, float& fyn, float& fzn, float alphan) {
+	if(ext & (EXT_VOLUME_FORCE | EXT_POROSITY)) {
+		const float rho2 = 0.5f / rhon;
+		ux = __builtin_fmaf(fxn, rho2, ux);
+		uy = __builtin_fmaf(fyn, rho2, uy);
+		uz = __builtin_fmaf(fzn, rho2, uz);
+		if(ext & EXT_POROSITY) {
+			const float k = 1.0f / (1.0f + 0.5f * alphan);
+			ux = ux * k; uy = uy * k; uz = uz * k;
+		}
+		ux = clampc(ux); uy = clampc(uy); uz = clampc(uz);
+		if(ext & EXT_POROSITY) {
+			const float ar = -(alphan * rhon);
+			fxn = __builtin_fmaf(ar, ux, fxn);
+			fyn = __builtin_fmaf(ar, uy, fyn);
+			fzn = __builtin_fmaf(ar, uz, fzn);
+		}
+	} else {
+		ux = clampc(ux); uy = clampc(uy); uz = clampc(uz);
+	}
+}
+
+// ------------------------------------------------------------------------------------
+// Lattice addressing for one lane: x of this lane, the uniform (y,z) row offsets.
+// ------------------------------------------------------------------------------------
+struct Site {
+	uint32_t X[3];   // x-1, x, x+1 (periodic), lane values
+	uint64_t R[9];   // row offsets (y+dy)*Nx + (z+dz)*Nx*Ny for dy,dz in {-1,0,1}, uniform
+	uint32_t x;
+	__device__ __forceinline__ uint64_t row(int dy, int dz) const { return R[(dy + 1) * 3 + (dz + 1)]; }
+	__device__ __forceinline__ uint64_t n() const { return R[4] + x; }
+};
+__device__ __forceinline__ Site make_site(uint32_t x, uint32_t y, uint32_t z, uint32_t Nx, uint32_t Ny, uint32_t Nz) {
+	Site s;
+	s.x = x;
+	s.X[0] = x == 0u ? Nx - 1u : x - 1u;
+	s.X[1] = x;
+	s.X[2] = x + 1u == Nx ? 0u : x + 1u;
+	const uint32_t Y[3] = {y == 0u ? Ny - 1u : y - 1u, y, y + 1u == Ny ? 0u : y + 1u};
+	const uint32_t Z[3] = {z == 0u ? Nz - 1u : z - 1u, z, z + 1u == Nz ? 0u : z + 1u};
+	const uint64_t NxNy = (uint64_t)Nx * Ny;
+#pragma unroll
+	for(int a = 0; a < 3; a++)
+#pragma unroll
+		for(int b = 0; b < 3; b++) s.R[a * 3 + b] = (uint64_t)Y[a] * Nx + (uint64_t)Z[b] * NxNy;
+	return s;
+}
+
+// Esoteric-Pull load_f / store_f (src/kernel.cpp:1326-1339).  For odd i the cell owns
+// slot pair (n, i|i+1) and (n+e_i, i+1|i); which one holds which population flips with
+// the parity of t.  Loads and stores touch the same locations, so the update is in place
+// and race-free.
+template<int Q, uint32_t ST>
+__device__ __forceinline__ void load_f(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, float* f) {
+	typedef Storage<ST> S;
+	const uint64_t n0 = s.R[4];
+	f[0] = S::load(fi + n0, s.x);
+#pragma unroll
+	for(int i = 1; i < Q; i += 2) {
+		const uint64_t sa = odd ? (uint64_t)i : (uint64_t)(i + 1);
+		const uint64_t sb = odd ? (uint64_t)(i + 1) : (uint64_t)i;
+		f[i] = S::load(fi + sa * N + n0, s.x);
+		f[i + 1] = S::load(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]), s.X[VS<Q>::cx[i] + 1]);
+	}
+}
+template<int Q, uint32_t ST>
+__device__ __forceinline__ void store_f(typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, const float* f) {
+	typedef Storage<ST> S;
+	const uint64_t n0 = s.R[4];
+	S::store(fi + n0, s.x, f[0]);
+#pragma unroll
+	for(int i = 1; i < Q; i += 2) {
+		const uint64_t sa = odd ? (uint64_t)i : (uint64_t)(i + 1);
+		const uint64_t sb = odd ? (uint64_t)(i + 1) : (uint64_t)i;
+		S::store(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]), s.X[VS<Q>::cx[i] + 1], f[i]);
+		S::store(fi + sa * N + n0, s.x, f[i + 1]);
+	}
+}
+
+// ------------------------------------------------------------------------------------
+// Kernels
+// ------------------------------------------------------------------------------------
+constexpr int BLOCK = 256;
+
+// initialize (src/kernel.cpp:1358-1430), non-moving boundaries
+template<int Q, uint32_t ST>
+__global__ __launch_bounds__(BLOCK) void k_initialize(LbmArgs a) {
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+	if(x >= a.Nx) return;
+	const Site s = make_site(x, blockIdx.y, blockIdx.z, a.Nx, a.Ny, a.Nz);
+	const uint64_t n = s.n();
+	float ux = a.u[n], uy = a.u[a.N + n], uz = a.u[2ull * a.N + n];
+	if((a.flags[n] & TYPE_BO) == TYPE_S) {
+		ux = 0.0f; uy = 0.0f; uz = 0.0f;
+		a.u[n] = 0.0f; a.u[a.N + n] = 0.0f; a.u[2ull * a.N + n] = 0.0f;
+	}
+	float feq[Q];
+	calc_feq<Q>(a.rho[n], ux, uy, uz, feq);
+	store_f<Q, ST>((typename Storage<ST>::E*)a.fi, a.N, s, 1u, feq);
+}
+
+// stream_collide (src/kernel.cpp:1454-1636)
+template<int Q, uint32_t ST, bool TRT>
+__global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+	if(x >= a.Nx) return;
+	const Site s = make_site(x, blockIdx.y, blockIdx.z, a.Nx, a.Ny, a.Nz);
+	const uint64_t n = s.n();
+	const uint32_t flagsn = a.flags[n];
+	const uint32_t bo = flagsn & TYPE_BO;
+	if(bo == TYPE_S || (flagsn & TYPE_SU) == TYPE_G) return; // solid cells are mailboxes
+	typedef typename Storage<ST>::E E;
+	E* fi = (E*)a.fi;
+	float fhn[Q];
+	load_f<Q, ST>(fi, a.N, s, a.odd, fhn);
+
+	const bool isE = (a.ext & EXT_EQUILIBRIUM_BOUNDARIES) && bo == TYPE_E;
+	float rhon, uxn, uyn, uzn;
+	calc_rho_u<Q>(fhn, rhon, uxn, uyn, uzn);
+	if(isE) { // preset density/velocity of equilibrium boundary cells
+		rhon = a.rho[n]; uxn = a.u[n]; uyn = a.u[a.N + n]; uzn = a.u[2ull * a.N + n];
+	}
+	float fxn = a.fx, fyn = a.fy, fzn = a.fz;
+	if(a.ext & EXT_FORCE_FIELD) { fxn += a.F[n]; fyn += a.F[a.N + n]; fzn += a.F[2ull * a.N + n]; }
+	const float alphan = (a.ext & EXT_POROSITY) ? a.alpha[n] : 0.0f;
+	force_block(a.ext, rhon, uxn, uyn, uzn, fxn, fyn, fzn, alphan);
+	const bool vf = a.ext & (EXT_VOLUME_FORCE | EXT_POROSITY);
+	float Fin[Q];
+	if(vf) calc_forcing<Q>(uxn, uyn, uzn, fxn, fyn, fzn, Fin);
+	else {
+#pragma unroll
+		for(int i = 0; i < Q; i++) Fin[i] = 0.0f;
+	}
+	if((a.ext & EXT_UPDATE_FIELDS) && !isE) {
+		a.rho[n] = rhon; a.u[n] = uxn; a.u[a.N + n] = uyn; a.u[2ull * a.N + n] = uzn;
+	}
+	float feq[Q];
+	calc_feq<Q>(rhon, uxn, uyn, uzn, feq);
+	const float w = a.w;
+	if constexpr(!TRT) { // src/kernel.cpp:1595-1604
+		if(vf) {
+			const float c_tau = __builtin_fmaf(w, -0.5f, 1.0f);
+#pragma unroll
+			for(int i = 0; i < Q; i++) Fin[i] *= c_tau;
+		}
+		const float omw = 1.0f - w;
+#pragma unroll
+		for(int i = 0; i < Q; i++) {
+			const float c = __builtin_fmaf(omw, fhn[i], __builtin_fmaf(w, feq[i], Fin[i]));
+			fhn[i] = isE ? feq[i] : c;
+		}
+	} else { // src/kernel.cpp:1605-1633, Lambda = 3/16
+		const float wp = w;
+		const float wm = 1.0f / (0.1875f / (1.0f / w - 0.5f) + 0.5f);
+		if(vf) {
+			const float c_taup = __builtin_fmaf(wp, -0.25f, 0.5f), c_taum = __builtin_fmaf(wm, -0.25f, 0.5f);
+			float Fn[Q];
+			Fn[0] = __builtin_fmaf(c_taup, Fin[0] + Fin[0], c_taum * (Fin[0] - Fin[0]));
+#pragma unroll
+			for(int i = 1; i < Q; i += 2) {
+				Fn[i] = __builtin_fmaf(c_taup, Fin[i] + Fin[i + 1], c_taum * (Fin[i] - Fin[i + 1]));
+				Fn[i + 1] = __builtin_fmaf(c_taup, Fin[i + 1] + Fin[i], c_taum * (Fin[i + 1] - Fin[i]));
+			}
+#pragma unroll
+			for(int i = 0; i < Q; i++) Fin[i] = Fn[i];
+		}
+		const float hwp = 0.5f * wp, hwm = 0.5f * wm;
+		float out[Q];
+#pragma unroll
+		for(int i = 0; i < Q; i++) {
+			const int b = i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); // opposite slot
+			const float c = __builtin_fmaf(hwp, ((feq[i] - fhn[i]) + feq[b]) - fhn[b],
+			                               __builtin_fmaf(hwm, ((feq[i] - feq[b]) - fhn[i]) + fhn[b], fhn[i] + Fin[i]));
+			out[i] = isE ? feq[i] : c;
+		}
+#pragma unroll
+		for(int i = 0; i < Q; i++) fhn[i] = out[i];
+	}
+	store_f<Q, ST>(fi, a.N, s, a.odd, fhn);
+}
+
+// update_fields (src/kernel.cpp:1794-1870)
+template<int Q, uint32_t ST>
+__global__ __launch_bounds__(BLOCK) void k_update_fields(LbmArgs a) {
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+	if(x >= a.Nx) return;
+	const Site s = make_site(x, blockIdx.y, blockIdx.z, a.Nx, a.Ny, a.Nz);
+	const uint64_t n = s.n();
+	const uint32_t flagsn = a.flags[n];
+	const uint32_t bo = flagsn & TYPE_BO;
+	if(bo == TYPE_S || (flagsn & TYPE_SU) == TYPE_G) return;
+	float fhn[Q];
+	load_f<Q, ST>((const typename Storage<ST>::E*)a.fi, a.N, s, a.odd, fhn);
+	float rhon, uxn, uyn, uzn;
+	calc_rho_u<Q>(fhn, rhon, uxn, uyn, uzn);
+	float fxn = a.fx, fyn = a.fy, fzn = a.fz;
+	if(a.ext & EXT_FORCE_FIELD) { fxn += a.F[n]; fyn += a.F[a.N + n]; fzn += a.F[2ull * a.N + n]; }
+	const float alphan = (a.ext & EXT_POROSITY) ? a.alpha[n] : 0.0f;
+	force_block(a.ext, rhon, uxn, uyn, uzn, fxn, fyn, fzn, alphan);
+	if(!((a.ext & EXT_EQUILIBRIUM_BOUNDARIES) && bo == TYPE_E)) {
+		a.rho[n] = rhon; a.u[n] = uxn; a.u[a.N + n] = uyn; a.u[2ull * a.N + n] = uzn;
+	}
+}
+
+// update_force_field (src/kernel.cpp:1873-1884): momentum exchange on solid cells
+template<int Q, uint32_t ST>
+__global__ __launch_bounds__(BLOCK) void k_update_force_field(LbmArgs a) {
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+	if(x >= a.Nx) return;
+	const Site s = make_site(x, blockIdx.y, blockIdx.z, a.Nx, a.Ny, a.Nz);
+	const uint64_t n = s.n();
+	if((a.flags[n] & TYPE_BO) != TYPE_S) return;
+	float fhn[Q];
+	load_f<Q, ST>((const typename Storage<ST>::E*)a.fi, a.N, s, a.odd, fhn);
+	float Fb, fxv, fyv, fzv;
+	calc_rho_u<Q>(fhn, Fb, fxv, fyv, fzv);
+	const float sc = 2.0f * Fb;
+	float* F = (float*)a.F;
+	F[n] = sc * fxv; F[a.N + n] = sc * fyv; F[2ull * a.N + n] = sc * fzv;
+}
+
+// Total mass, per-row partial sums (FP64): every cell, solid or not, contributes 1 plus
+// the shifted DDFs stored in its own slots fi[i*N+n].  That sum is invariant under
+// stream_collide (values only move inside disjoint footprints, collisions conserve
+// sum_i f_i); one partial per block, summed later in a fixed order.
+template<int Q, uint32_t ST>
+__global__ __launch_bounds__(BLOCK) void k_mass_partial(LbmArgs a, double* partial) {
+	__shared__ double red[BLOCK];
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+	double v = 0.0;
+	if(x < a.Nx) {
+		const uint64_t n = ((uint64_t)blockIdx.z * a.Ny + blockIdx.y) * a.Nx + x;
+		const typename Storage<ST>::E* fi = (const typename Storage<ST>::E*)a.fi;
+		v = 1.0;
+#pragma unroll
+		for(int i = 0; i < Q; i++) v += (double)Storage<ST>::load(fi + (uint64_t)i * a.N + n, 0u);
+	}
+	red[threadIdx.x] = v;
+	__syncthreads();
+	for(uint32_t k = blockDim.x / 2u; k > 0u; k >>= 1) { // blockDim.x is 64, 128 or 256
+		if(threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+		__syncthreads();
+	}
+	if(threadIdx.x == 0) partial[((uint64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+} // namespace fx

@@ -1,0 +1,198 @@
+/*
+ * fx_lbm.h - C ABI of the MI355X-native lattice Boltzmann core (libfx_lbm.so).
+ *
+ * This layer replaces FluidX3D's OpenCL-Wrapper device/buffer/kernel layer
+ * (src/opencl.hpp:284-692) for the LBM hot path.  Everything is plain C: opaque context
+ * handles, device pointers as void*, sizes as uint64_t, int status codes.  No torch
+ * types.  Citations are paths in the FluidX3D v3.7 reference repository.
+ *
+ * Conventions
+ *   - Every function returns FX_OK (0) or a negative FX_ERR_* code.  The message of the
+ *     last failure on the calling thread is returned by fx_last_error().  The reference
+ *     calls print_error()+exit(1) instead (src/utilities.hpp:4074-4086); the C++ host
+ *     layer (fluidx3d_amd/host) maps a nonzero status back to that behaviour.
+ *   - Device arrays use the reference's layout: fi[i*N+n] (SoA DDFs, i = slot, n =
+ *     x+(y+z*Ny)*Nx), rho[n], u[d*N+n], flags[n] (uint8 TYPE_* bits), F[d*N+n],
+ *     alpha[n] (src/kernel.cpp:861-863, 960-967, src/lbm.cpp:121-176).
+ *   - Kernel entry points are asynchronous on the context's HIP stream, in submission
+ *     order, like the reference's in-order command queue (src/opencl.hpp:311).  Entry
+ *     points that return host values (fx_lbm_object_*, fx_lbm_mass_sum, timing) wait
+ *     for their own work.  A context is not re-entrant; separate contexts are
+ *     independent.
+ */
+#ifndef FX_LBM_H
+#define FX_LBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FX_OK 0
+#define FX_ERR_ARG (-1)      /* invalid argument (null pointer, bad size, bad enum) */
+#define FX_ERR_HIP (-2)      /* HIP runtime error; fx_last_error() has the HIP message */
+#define FX_ERR_CONFIG (-3)   /* configuration rejected (grid size 0, nu <= 0, ...) */
+#define FX_ERR_MEMORY (-4)   /* allocation failed or grid does not fit device memory */
+#define FX_ERR_STATE (-5)    /* call not valid in the current state */
+
+/* velocity sets (src/defines.hpp:5-8), collision operators (:10-11), DDF storage (:13-14) */
+#define FX_D3Q19 19
+#define FX_D3Q27 27
+#define FX_SRT 0
+#define FX_TRT 1
+#define FX_FP32 0
+#define FX_FP16S 1
+#define FX_FP16C 2
+
+/* extension bits (src/defines.hpp:18-25); FX_POROSITY is new: a per-cell Brinkman
+ * porosity alpha[n] >= 0 folded into the Guo force (u = (j/rho + f/(2 rho))/(1+alpha/2),
+ * F = f - alpha*rho*u).  With alpha == 0 it is bit-identical to FX_VOLUME_FORCE. */
+#define FX_VOLUME_FORCE 1u
+#define FX_FORCE_FIELD 2u
+#define FX_EQUILIBRIUM_BOUNDARIES 4u
+#define FX_UPDATE_FIELDS 8u
+#define FX_POROSITY 16u
+
+/* cell type bits (src/defines.hpp:53-60) */
+#define FX_TYPE_S 0x01u
+#define FX_TYPE_E 0x02u
+#define FX_TYPE_T 0x04u
+#define FX_TYPE_F 0x08u
+#define FX_TYPE_I 0x10u
+#define FX_TYPE_G 0x20u
+#define FX_TYPE_X 0x40u
+#define FX_TYPE_Y 0x80u
+
+/* One lattice configuration: what the reference bakes into the OpenCL source as
+ * #defines (src/lbm.cpp:334-468).  Here all variants are compiled ahead of time for
+ * gfx950 and the context selects one. */
+typedef struct fx_lbm_config {
+    uint32_t Nx, Ny, Nz;     /* grid size; N = Nx*Ny*Nz cells */
+    uint32_t velocity_set;   /* FX_D3Q19 or FX_D3Q27 */
+    uint32_t collision;      /* FX_SRT or FX_TRT */
+    uint32_t storage;        /* FX_FP32, FX_FP16S or FX_FP16C */
+    uint32_t extensions;     /* OR of FX_VOLUME_FORCE, ... */
+    float w;                 /* relaxation rate 1/tau as the kernels use it; use
+                                fx_relaxation_rate(nu) for the reference's value */
+} fx_lbm_config;
+
+typedef struct fx_device_info {
+    char name[256];          /* marketing name */
+    char arch[64];           /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+    uint64_t memory_bytes;   /* global memory */
+    uint64_t memory_free;    /* free global memory now */
+    uint32_t compute_units;
+    uint32_t clock_mhz;
+} fx_device_info;
+
+typedef struct fx_ctx fx_ctx;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fx_last_error(void);
+
+/* Device discovery; replaces Device_Info / get_devices (src/opencl.hpp:89-252). */
+int fx_device_count(int* count);
+int fx_device_info_get(int device, fx_device_info* out);
+
+/* Relaxation rate w = 1/tau, tau = 3*nu+1/2 (src/lbm.hpp:132), after the reference's
+ * decimal round trip: to_string(1.0f/tau) is printed with 9 significant digits and
+ * re-parsed as a float literal by the device compiler (src/lbm.cpp:367,
+ * src/utilities.hpp:2599-2621, 2745-2754). */
+float fx_relaxation_rate(float nu);
+
+/* Byte accounting per cell: device memory footprint (bytes_per_cell_device,
+ * src/lbm.cpp:38-50) and algorithmic HBM traffic per cell and time step
+ * (bandwidth_bytes_per_cell_device, src/lbm.cpp:51-72).  0 for an invalid config. */
+uint64_t fx_memory_bytes_per_cell(const fx_lbm_config* cfg);
+uint64_t fx_bandwidth_bytes_per_cell(const fx_lbm_config* cfg);
+
+/* Context = device + one in-order HIP stream + the selected precompiled kernel variant.
+ * Replaces Device(info, opencl_c_code) (src/opencl.hpp:284-340) and the per-domain
+ * kernel set of LBM_Domain (src/lbm.cpp:96-176).  Validates the configuration like
+ * LBM::sanity_checks_constructor (src/lbm.cpp:783-838) but returns an error code. */
+int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out);
+int fx_ctx_destroy(fx_ctx* ctx);
+int fx_ctx_config(const fx_ctx* ctx, fx_lbm_config* out);
+/* Wait for all work on the context stream; replaces Device::finish_queue (src/opencl.hpp:335). */
+int fx_sync(fx_ctx* ctx);
+
+/* Device buffers; replace the device half of Memory<T> (src/opencl.hpp:342-614).
+ * Host memory stays with the caller.  fx_buf_write/read copy between a host pointer and
+ * dst/src+offset; blocking != 0 waits for the copy (enqueue_write_to_device /
+ * enqueue_read_from_device, src/opencl.hpp:510-531).  fx_buf_fill sets every byte.
+ * fx_buf_copy is device to device on the context stream. */
+int fx_buf_alloc(fx_ctx* ctx, uint64_t bytes, void** dptr);
+int fx_buf_free(fx_ctx* ctx, void* dptr);
+int fx_buf_write(fx_ctx* ctx, void* dst, uint64_t offset, uint64_t bytes, const void* host, int blocking);
+int fx_buf_read(fx_ctx* ctx, const void* src, uint64_t offset, uint64_t bytes, void* host, int blocking);
+int fx_buf_fill(fx_ctx* ctx, void* dst, uint64_t bytes, uint8_t value);
+int fx_buf_copy(fx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---- LBM kernels (typed launches; replace Kernel::enqueue_run, src/opencl.hpp:616-692) ---- */
+
+/* initialize (src/kernel.cpp:1358-1430): zero u of TYPE_S cells, store feq(rho,u) of
+ * every cell with the odd-step Esoteric-Pull convention (t=1).  Bound as
+ * kernel_initialize(fi,rho,u,flags) in src/lbm.cpp:127; the host sequencing
+ * (t=1 then t=0, src/lbm.cpp:881-922) is the caller's. */
+int fx_lbm_initialize(fx_ctx* ctx, void* fi, const float* rho, float* u, const uint8_t* flags);
+
+/* stream_collide (src/kernel.cpp:1454-1636): one time step t, in place (Esoteric-Pull).
+ * F (3N floats) is read only with FX_FORCE_FIELD, alpha (N floats) only with
+ * FX_POROSITY, rho/u only for TYPE_E cells with FX_EQUILIBRIUM_BOUNDARIES and written
+ * with FX_UPDATE_FIELDS.  Replaces LBM_Domain::enqueue_stream_collide (src/lbm.cpp:181-183). */
+int fx_lbm_stream_collide(fx_ctx* ctx, void* fi, float* rho, float* u, const uint8_t* flags,
+                          uint64_t t, float fx, float fy, float fz, const float* F, const float* alpha);
+
+/* steps consecutive stream_collide launches for t = t0 .. t0+steps-1, bracketed by HIP
+ * events on the context stream; waits and returns the elapsed milliseconds in *ms
+ * (may be NULL: then it does not wait).  Replaces the loop of LBM::run
+ * (src/lbm.cpp:955-975) without the per-step host synchronisation of src/lbm.cpp:951. */
+int fx_lbm_run(fx_ctx* ctx, void* fi, float* rho, float* u, const uint8_t* flags, uint64_t t0, uint64_t steps,
+               float fx, float fy, float fz, const float* F, const float* alpha, float* ms);
+
+/* update_fields (src/kernel.cpp:1794-1870): rho/u of non-solid cells from the DDFs at
+ * time t (TYPE_E cells keep theirs with FX_EQUILIBRIUM_BOUNDARIES).  Replaces
+ * LBM_Domain::enqueue_update_fields (src/lbm.cpp:184-191). */
+int fx_lbm_update_fields(fx_ctx* ctx, const void* fi, float* rho, float* u, const uint8_t* flags,
+                         uint64_t t, float fx, float fy, float fz, const float* F, const float* alpha);
+
+/* update_force_field (src/kernel.cpp:1873-1884): F = momentum exchange on TYPE_S cells at
+ * time t.  reset_force_field (:1885-1889) zeroes F.  Replaces
+ * LBM_Domain::enqueue_update_force_field (src/lbm.cpp:207-212). */
+int fx_lbm_update_force_field(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t t, float* F);
+int fx_lbm_reset_force_field(fx_ctx* ctx, float* F);
+
+/* Object sums over cells with flags[n]==marker (src/kernel.cpp:1901-1959; host
+ * src/lbm.cpp:213-238, 986-1016).  Per-cell terms in FP32 as in the reference, summed
+ * in FP64 in a fixed order (deterministic, unlike the reference's float atomics).
+ * Synchronous.  center_of_mass writes (sum x, sum y, sum z, cell count). */
+int fx_lbm_object_force(fx_ctx* ctx, const float* F, const uint8_t* flags, uint8_t marker, double out[3]);
+int fx_lbm_object_torque(fx_ctx* ctx, const float* F, const uint8_t* flags, uint8_t marker,
+                         float cx, float cy, float cz, double out[3]);
+int fx_lbm_object_center_of_mass(fx_ctx* ctx, const uint8_t* flags, uint8_t marker, double out[4]);
+
+/* Total mass: sum over all cells (solid ones included) of 1 + the shifted DDFs stored in
+ * the cell's own slots, FP64, deterministic.  This is exactly invariant under
+ * stream_collide (values only move inside disjoint Esoteric-Pull footprints and the
+ * collision conserves sum_i f_i); populations in flight toward a wall are held in the
+ * solid cell's slots.  t is accepted for symmetry and not used.  No reference
+ * counterpart; the conservation check. */
+int fx_lbm_mass_sum(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t t, double* out);
+
+/* ---- diagnostics ---- */
+
+/* Storage codec round trip on the device: codes[k] = encode(x[k]), decoded[k] =
+ * decode(codes[k]) for storage FX_FP16S or FX_FP16C, exactly as the kernels store and
+ * load DDFs (src/lbm.cpp:410-414, src/kernel.cpp:848-859).  Device pointers. */
+int fx_codec_roundtrip(fx_ctx* ctx, uint32_t storage, const float* x, uint16_t* codes, float* decoded, uint64_t n);
+
+/* Device-to-device copy kernel over bytes (multiple of 16) repeated reps times,
+ * timed with HIP events: the measured HBM peak and the counter calibration. */
+int fx_bench_copy(fx_ctx* ctx, void* dst, const void* src, uint64_t bytes, uint32_t reps, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FX_LBM_H */

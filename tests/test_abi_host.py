@@ -1,0 +1,78 @@
+"""The C ABI library on a machine without a GPU: it loads, exports exactly what
+include/fx_lbm.h declares, and its host-side logic (relaxation rate, byte accounting,
+argument validation) behaves like the reference's host code.  No kernel is launched."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "fx_lbm.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(fx_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from fluidx3d_amd import _abi
+    L = _abi.load()
+    declared = header_functions()
+    assert len(declared) >= 25
+    assert sorted(_abi.EXPORTS) == declared
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+def test_byte_accounting_matches_reference():
+    """bandwidth_bytes_per_cell_device / bytes_per_cell_device, src/lbm.cpp:25-72."""
+    import fluidx3d_amd as fx
+    bw = lambda q, st, ext=(): fx.bandwidth_bytes_per_cell(fx.make_config(8, 8, 8, 1.0, q, "SRT", st, ext))
+    mem = lambda q, st, ext=(): fx.memory_bytes_per_cell(fx.make_config(8, 8, 8, 1.0, q, "SRT", st, ext))
+    assert bw(19, "FP32") == 153 and bw(19, "FP16S") == 77 and bw(19, "FP16C") == 77
+    assert bw(27, "FP32") == 217 and bw(27, "FP16C") == 109
+    assert bw(19, "FP32", ("UPDATE_FIELDS",)) == 169 and bw(19, "FP32", ("FORCE_FIELD",)) == 165
+    assert mem(19, "FP32") == 93 and mem(19, "FP16S") == 55 and mem(27, "FP32") == 125 and mem(27, "FP16C") == 71
+    assert mem(19, "FP32", ("FORCE_FIELD",)) == 105
+
+
+def test_config_errors_are_status_codes():
+    from fluidx3d_amd import _abi
+    L = _abi.load()
+    h = ctypes.c_void_p()
+    bad = [
+        _abi.make_config(0, 8, 8, 1.0),                  # zero cells
+        _abi.make_config(8, 8, 8, 0.0),                  # w = 0 (nu -> infinity)
+        _abi.make_config(8, 8, 8, 2.5),                  # w > 2 (nu < 0)
+        _abi.Config(8, 8, 8, 15, 0, 0, 0, 1.0),          # unsupported velocity set
+        _abi.Config(8, 8, 8, 19, 0, 7, 0, 1.0),          # unsupported storage
+        _abi.Config(8, 8, 8, 19, 0, 0, 1 << 9, 1.0),     # unknown extension bit
+    ]
+    for cfg in bad:
+        rc = L.fx_ctx_create(0, ctypes.byref(cfg), ctypes.byref(h))
+        assert rc == -3, rc
+        assert L.fx_last_error().decode()
+        assert not h.value
+    assert L.fx_ctx_create(0, None, ctypes.byref(h)) == -1
+    assert L.fx_lbm_stream_collide(None, None, None, None, None, 0, 0.0, 0.0, 0.0, None, None) == -1
+
+
+def test_python_host_checks_like_reference():
+    """LBM::sanity_checks_constructor (src/lbm.cpp:802-813) raise before touching a device."""
+    import fluidx3d_amd as fx
+    with pytest.raises(ValueError):
+        fx.LBM(8, 8, 8, 0.0)
+    with pytest.raises(ValueError):
+        fx.LBM(8, 8, 8, -0.1)
+    with pytest.raises(ValueError):
+        fx.LBM(8, 8, 8, 0.1, 1e-5, 0.0, 0.0)
+
+
+def test_relaxation_rate_known_values(oracle):
+    import fluidx3d_amd as fx
+    assert fx.relaxation_rate(1.0 / 6.0) == 1.0
+    assert fx.relaxation_rate(0.048) == np.float32(1.55279504)
+    for nu in (0.01, 0.2, 1.0, 3.3e-3):
+        assert fx.relaxation_rate(nu) == oracle.relaxation_rate(nu)

@@ -1,0 +1,212 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle on the same inputs.
+
+Bar: FP32 arithmetic is the reference's operation order on both sides, so DDFs and
+rho/u agree exactly (as float values; the sign of an exact zero is not compared, see
+fluidx3d_amd/csrc/lbm_device.hpp).  FP16S/FP16C DDFs agree code-for-code up to the sign
+of zero.  Each FP32 result's error against the FP64 oracle is stated in the TGV test.
+"""
+import ctypes
+import itertools
+
+import numpy as np
+import pytest
+
+from tests import scenes
+
+pytestmark = pytest.mark.gpu
+
+EXT_SETS = {
+    "plain": (),
+    "forces": ("VOLUME_FORCE", "FORCE_FIELD", "EQUILIBRIUM_BOUNDARIES", "UPDATE_FIELDS"),
+    "porosity": ("POROSITY",),
+}
+EXT_BITS = {"VOLUME_FORCE": 1, "FORCE_FIELD": 2, "EQUILIBRIUM_BOUNDARIES": 4, "UPDATE_FIELDS": 8, "POROSITY": 16}
+
+
+def ext_bits(names):
+    b = 0
+    for n in names:
+        b |= EXT_BITS[n]
+    return b
+
+
+def decoded(storage, ddf, oracle):
+    if storage == "FP32":
+        return ddf.astype(np.float32)
+    return oracle.decode(storage, ddf)
+
+
+def setup_pair(oracle, Nx, Ny, Nz, nu, q, collision, storage, ext_names, rho, u, flags, F=None, alpha=None, f=(0.0, 0.0, 0.0)):
+    import fluidx3d_amd as fx
+    lbm = fx.LBM(Nx, Ny, Nz, nu, *f, velocity_set=q, collision=collision, storage=storage, extensions=ext_names)
+    ref = oracle.OracleLBM(Nx, Ny, Nz, w=lbm.w, q=q, trt=(collision == "TRT"), storage=storage, ext=ext_bits(ext_names),
+                           fx=np.float32(f[0]), fy=np.float32(f[1]), fz=np.float32(f[2]))
+    for tgt in (lbm, ref):
+        r = tgt.rho.host if hasattr(tgt.rho, "host") else tgt.rho
+        v = tgt.u.host if hasattr(tgt.u, "host") else tgt.u
+        fl = tgt.flags.host if hasattr(tgt.flags, "host") else tgt.flags
+        r[:] = rho
+        v[:] = u
+        fl[:] = flags
+    if F is not None:
+        lbm.F.host[:] = F
+        ref.F[:] = F
+    if alpha is not None:
+        lbm.alpha.host[:] = alpha
+        ref.alpha[:] = alpha
+    return lbm, ref
+
+
+def compare_state(lbm, ref, oracle, storage, tag):
+    g = decoded(storage, lbm.read_ddfs(), oracle)
+    o = decoded(storage, ref.fi, oracle)
+    bad = np.flatnonzero(g != o)
+    assert bad.size == 0, f"{tag}: {bad.size} DDF values differ, first slot/cell {divmod(int(bad[0]), lbm.N)}: gpu {g[bad[0]]!r} oracle {o[bad[0]]!r}"
+
+
+@pytest.mark.parametrize("q,collision,storage,ext", list(itertools.product((19, 27), ("SRT", "TRT"), ("FP32", "FP16S", "FP16C"), EXT_SETS)))
+def test_stream_collide_matches_oracle(oracle, q, collision, storage, ext):
+    """Random fields with scattered solids (and TYPE_E planes, forces, porosity): 7 steps,
+    both parities, ragged x-rows (Nx=20 in a 64-lane block)."""
+    Nx, Ny, Nz = 20, 12, 10
+    names = EXT_SETS[ext]
+    rho, u, flags = scenes.random_scene(Nx, Ny, Nz, seed=q + len(names), equilibrium_faces="EQUILIBRIUM_BOUNDARIES" in names)
+    N = Nx * Ny * Nz
+    rng = np.random.default_rng(7)
+    F = (1e-4 * rng.uniform(-1, 1, 3 * N)).astype(np.float32) if "FORCE_FIELD" in names else None
+    alpha = rng.uniform(0, 1, N).astype(np.float32) if "POROSITY" in names else None
+    f = (2e-5, -1e-5, 3e-6) if ("VOLUME_FORCE" in names or "POROSITY" in names) else (0.0, 0.0, 0.0)
+    lbm, ref = setup_pair(oracle, Nx, Ny, Nz, 0.048, q, collision, storage, names, rho, u, flags, F, alpha, f)
+    try:
+        lbm.initialize()
+        ref.initialize()
+        compare_state(lbm, ref, oracle, storage, "after initialize")
+        for s in range(7):
+            lbm.run(1)
+            ref.step(1)
+            compare_state(lbm, ref, oracle, storage, f"after step {s + 1}")
+        lbm.u.read_from_device()
+        lbm.rho.read_from_device()
+        if "UPDATE_FIELDS" not in names:  # with UPDATE_FIELDS rho/u come from the last stream_collide
+            ref.update_fields()           # (src/lbm.cpp:184-191, src/lbm.hpp:390-393)
+        assert np.array_equal(lbm.rho.host, ref.rho), "rho differs after update_fields"
+        assert np.array_equal(lbm.u.host, ref.u), "u differs after update_fields"
+        assert abs(lbm.mass() - ref.mass()) <= 1e-12 * abs(ref.mass())
+    finally:
+        lbm.close()
+
+
+@pytest.mark.parametrize("storage", ["FP32", "FP16S", "FP16C"])
+def test_tgv_64_1000_steps(oracle, storage):
+    """Config C1 shape: D3Q19 SRT 64^3 Taylor-Green, nu=1/6, u0=0.05, 1000 steps.
+    GPU == FP32 oracle (same storage) exactly; the error against the FP64 oracle is stated
+    and bounded relative to u0; the kinetic energy decays at the viscous rate 2*nu*|k|^2."""
+    Nx = Ny = Nz = 64
+    N = Nx * Ny * Nz
+    u0 = 0.05
+    rho, u, flags = scenes.taylor_green(Nx, Ny, Nz, u0=u0)
+    lbm, ref = setup_pair(oracle, Nx, Ny, Nz, 1.0 / 6.0, 19, "SRT", storage, (), rho, u, flags)
+    ref64 = oracle.OracleLBM(Nx, Ny, Nz, w=lbm.w, storage="FP64")
+    ref64.rho[:] = rho
+    ref64.u[:] = u
+    try:
+        lbm.initialize(); ref.initialize(); ref64.initialize()
+        lbm.run(200)
+        lbm.u.read_from_device(); lbm.rho.read_from_device()
+        ke200 = scenes.kinetic_energy(lbm.rho.host, lbm.u.host, N)
+        lbm.run(800); ref.step(1000); ref64.step(1000)
+        compare_state(lbm, ref, oracle, storage, "TGV 1000 steps")
+        lbm.u.read_from_device(); lbm.rho.read_from_device()
+        ref.update_fields(); ref64.update_fields()
+        assert np.array_equal(lbm.u.host, ref.u) and np.array_equal(lbm.rho.host, ref.rho)
+        err_u = np.abs(lbm.u.host.astype(np.float64) - ref64.u).max()
+        err_rho = np.abs(lbm.rho.host.astype(np.float64) - ref64.rho).max()
+        print(f"\n{storage}: max|u-u64| = {err_u:.3e} ({err_u / u0:.2e} u0), max|rho-rho64| = {err_rho:.3e}")
+        bound = {"FP32": 1e-5, "FP16S": 5e-5, "FP16C": 5e-5}[storage]
+        assert err_u < bound * u0 and err_rho < 1e-5
+        ke1000 = scenes.kinetic_energy(lbm.rho.host, lbm.u.host, N)
+        rate = np.log(ke200 / ke1000) / 800.0
+        expect = 2.0 * (1.0 / 6.0) * 3.0 * (2.0 * np.pi / 64.0) ** 2
+        assert abs(rate / expect - 1.0) < 0.01, (rate, expect)
+    finally:
+        lbm.close()
+
+
+def test_full_size_256_two_steps(oracle):
+    """Config C2 size: 256^3 D3Q19 SRT FP32 TGV, 2 steps, every cell compared with the oracle."""
+    Nx = Ny = Nz = 256
+    rho, u, flags = scenes.taylor_green(Nx, Ny, Nz)
+    lbm, ref = setup_pair(oracle, Nx, Ny, Nz, 1.0 / 6.0, 19, "SRT", "FP32", (), rho, u, flags)
+    try:
+        lbm.initialize(); ref.initialize()
+        lbm.run(2); ref.step(2)
+        compare_state(lbm, ref, oracle, "FP32", "256^3 2 steps")
+    finally:
+        lbm.close()
+
+
+@pytest.mark.parametrize("storage", ["FP16S", "FP16C"])
+def test_codec_matches_oracle(oracle, storage):
+    """Device encode/decode against the oracle: every 16-bit code that encodes a finite
+    value round-trips, and 2M random floats across all magnitudes encode identically."""
+    import fluidx3d_amd as fx
+    from fluidx3d_amd import _abi
+    cfg = fx.make_config(4, 4, 4, 1.0)
+    ctx = _abi.Context(cfg)
+    try:
+        codes = np.arange(65536, dtype=np.uint16)
+        vals = oracle.decode(storage, codes)
+        finite = np.isfinite(vals)
+        rng = np.random.default_rng(3)
+        rnd = rng.integers(0, 2 ** 32, 2 ** 21, dtype=np.uint64).astype(np.uint32).view(np.float32)
+        rnd = rnd[np.isfinite(rnd) & (np.abs(rnd) < (2.0 if storage == "FP16C" else 1.99))]
+        x = np.concatenate([vals[finite], rnd, (rng.uniform(-1, 1, 2 ** 20) * 10.0 ** rng.uniform(-12, 0, 2 ** 20)).astype(np.float32)])
+        n = x.size
+        dx = ctx.alloc(4 * n); dc = ctx.alloc(2 * n); dd = ctx.alloc(4 * n)
+        ctx.write(dx, x)
+        _abi.check(ctx.lib.fx_codec_roundtrip(ctx.h, _abi.STORAGE[storage], dx, dc, dd, n))
+        gc = np.empty(n, dtype=np.uint16); gd = np.empty(n, dtype=np.float32)
+        ctx.read(dc, gc); ctx.read(dd, gd)
+        oc = oracle.encode(storage, x)
+        od = oracle.decode(storage, oc)
+        mism = np.flatnonzero(gc != oc)
+        # only signed zeros may differ
+        assert np.all(od[mism] == 0) and np.all(gd[mism] == 0), f"{mism.size} codes differ, e.g. x={x[mism[:3]]}"
+        assert np.array_equal(gd, od)
+        m = finite.sum()
+        assert np.array_equal(gd[:m], vals[finite]), "decode(encode(decode(code))) != decode(code)"
+    finally:
+        ctx.close()
+
+
+def test_object_force_and_mass(oracle):
+    """FORCE_FIELD: momentum-exchange force field equal to the oracle's, FP64 object sums
+    equal to the oracle's FP64 sums to 1e-12."""
+    Nx, Ny, Nz = 48, 32, 32
+    flags = scenes.sphere_flags(Nx, Ny, Nz, (16, 16, 16), 6, marker=scenes.TYPE_S | scenes.TYPE_X)
+    x, _, _ = scenes.coords(Nx, Ny, Nz)
+    flags[(x == 0) | (x == Nx - 1)] = scenes.TYPE_E
+    N = Nx * Ny * Nz
+    rho = np.ones(N, dtype=np.float32)
+    u = np.zeros(3 * N, dtype=np.float32)
+    u[:N][(flags & scenes.TYPE_S) == 0] = 0.05
+    names = ("FORCE_FIELD", "EQUILIBRIUM_BOUNDARIES", "VOLUME_FORCE")
+    lbm, ref = setup_pair(oracle, Nx, Ny, Nz, 0.05, 27, "TRT", "FP32", names, rho, u, flags, F=np.zeros(3 * N, np.float32))
+    try:
+        lbm.initialize(); ref.initialize()
+        lbm.run(50); ref.step(50)
+        marker = scenes.TYPE_S | scenes.TYPE_X
+        g = lbm.object_force(marker)
+        o = ref.object_force(marker)
+        lbm.F.read_from_device()
+        assert np.array_equal(lbm.F.host, ref.F)
+        assert np.allclose(g, o, rtol=1e-12, atol=1e-15), (g, o)
+        assert g[0] > 0  # drag along the flow
+        c = (0.5 * Nx - 0.5, 0.5 * Ny - 0.5, 0.5 * Nz - 0.5)
+        assert np.allclose(lbm.object_torque(c, marker), ref.object_torque(c, marker), rtol=1e-10, atol=1e-12)
+        gc = lbm.object_center_of_mass(marker)
+        oc, cells = ref.object_center_of_mass(marker)
+        assert np.allclose(gc, oc, rtol=0, atol=1e-9)
+        assert abs(lbm.mass() - ref.mass()) <= 1e-12 * ref.mass()
+    finally:
+        lbm.close()
