@@ -295,10 +295,17 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 	ctx->device = device;
 	ctx->cfg = *cfg;
 	ctx->N = N;
-	ctx->k = cfg->velocity_set == FX_D3Q19 ? fx::kernels_q19(cfg->storage, cfg->collision == FX_TRT)
-	                                        : fx::kernels_q27(cfg->storage, cfg->collision == FX_TRT);
+	const bool plain = cfg->extensions == 0u;
+	ctx->k = cfg->velocity_set == FX_D3Q19 ? fx::kernels_q19(cfg->storage, cfg->collision == FX_TRT, plain)
+	                                        : fx::kernels_q27(cfg->storage, cfg->collision == FX_TRT, plain);
 	ctx->shape = fx::lattice_shape(cfg->Nx, cfg->Ny, cfg->Nz);
 	hipError_t e = hipSetDevice(device);
+	if(e == hipSuccess) {
+		hipDeviceProp_t prop;
+		e = hipGetDeviceProperties(&prop, device);
+		if(e == hipSuccess && ctx->k->stream_collide_waves_cap)
+			ctx->shape.lds = fx::occupancy_lds(ctx->k->stream_collide_waves_cap, ctx->shape.block.x, prop.maxSharedMemoryPerMultiProcessor, prop.sharedMemPerBlock);
+	}
 	if(e == hipSuccess) {
 		size_t free_b = 0, total_b = 0;
 		e = hipMemGetInfo(&free_b, &total_b);

@@ -48,11 +48,25 @@ struct LbmArgs {
 // ------------------------------------------------------------------------------------
 // DDF storage formats
 // ------------------------------------------------------------------------------------
+// Element o of the array at the wave-uniform base p, addressed as base + 32-bit byte
+// offset: the form global_load/global_store take with a scalar base register (saddr) and
+// one 32-bit VGPR offset per lane.  Callers keep o*sizeof(E) below 2^32 (rows of at most
+// 2^26 cells).
+template<typename E> __device__ __forceinline__ const E* elem(const E* p, uint32_t o) {
+	return (const E*)((const char*)p + (uint32_t)(o * (uint32_t)sizeof(E)));
+}
+template<typename E> __device__ __forceinline__ E* elem(E* p, uint32_t o) {
+	return (E*)((char*)p + (uint32_t)(o * (uint32_t)sizeof(E)));
+}
+
 template<uint32_t ST> struct Storage;
+// FP32 DDFs are streamed with non-temporal loads and stores: no DDF value is touched
+// twice in a step, and on MI355X this measured faster for 4-byte DDFs (while it is slower
+// for the 2-byte formats, which keep regular loads).
 template<> struct Storage<ST_FP32> {
 	typedef float E;
-	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return p[o]; }
-	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { p[o] = x; }
+	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
+	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { __builtin_nontemporal_store(x, elem(p, o)); }
 };
 // FP16S (src/lbm.cpp:410-414): IEEE binary16 of x*2^15, round to nearest even
 // (v_cvt_f16_f32 in the default rounding mode), load = binary16 * 2^-15.
@@ -60,8 +74,8 @@ template<> struct Storage<ST_FP16S> {
 	typedef uint16_t E;
 	__device__ static __forceinline__ float decode(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h) * 3.0517578E-5f; }
 	__device__ static __forceinline__ uint16_t encode(float x) { return __builtin_bit_cast(uint16_t, (_Float16)(x * 32768.0f)); }
-	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(p[o]); }
-	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { p[o] = encode(x); }
+	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(*elem(p, o)); }
+	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { *elem(p, o) = encode(x); }
 };
 // FP16C: 1 sign, 4 exponent (bias 15), 11 mantissa bits (src/kernel.cpp:848-859).
 // Decode without a leading-zero count: a normal code is the top bits of a binary32 with
@@ -87,8 +101,8 @@ template<> struct Storage<ST_FP16C> {
 		const uint32_t mag = e > 112u ? normal : (e > 100u ? denormal : 0u);
 		return (uint16_t)(((b >> 16) & 0x8000u) | mag);
 	}
-	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(p[o]); }
-	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { p[o] = encode(x); }
+	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(*elem(p, o)); }
+	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { *elem(p, o) = encode(x); }
 };
 
 // ------------------------------------------------------------------------------------
@@ -296,9 +310,12 @@ __global__ __launch_bounds__(BLOCK) void k_initialize(LbmArgs a) {
 	store_f<Q, ST>((typename Storage<ST>::E*)a.fi, a.N, s, 1u, feq);
 }
 
-// stream_collide (src/kernel.cpp:1454-1636)
-template<int Q, uint32_t ST, bool TRT>
+// stream_collide (src/kernel.cpp:1454-1636).  EXT is a compile-time mask of the
+// extensions this instantiation may use; the runtime mask a.ext selects among them.
+// EXT = 0 is the plain scheme with every extension path compiled out.
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
+	a.ext &= EXT;
 	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
 	if(x >= a.Nx) return;
 	const Site s = make_site(x, blockIdx.y, blockIdx.z, a.Nx, a.Ny, a.Nz);
@@ -377,8 +394,9 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
 }
 
 // update_fields (src/kernel.cpp:1794-1870)
-template<int Q, uint32_t ST>
+template<int Q, uint32_t ST, uint32_t EXT>
 __global__ __launch_bounds__(BLOCK) void k_update_fields(LbmArgs a) {
+	a.ext &= EXT;
 	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
 	if(x >= a.Nx) return;
 	const Site s = make_site(x, blockIdx.y, blockIdx.z, a.Nx, a.Ny, a.Nz);
