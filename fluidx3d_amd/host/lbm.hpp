@@ -1,0 +1,248 @@
+#pragma once
+// The LBM host class with the reference's interface (src/lbm.hpp:205-611) on top of the
+// C ABI in include/fx_lbm.h.  One lattice on one GPU: the reference's multi-domain
+// constructor accepts only Dx=Dy=Dz=1 here.
+#include "defines.hpp"
+#include "utilities.hpp"
+#include "units.hpp"
+#include "shapes.hpp"
+#include "info.hpp"
+#include "fx_lbm.h"
+
+uint bytes_per_cell_host();             // rho, u, flags (+F, +porosity), src/lbm.cpp:25-37
+uint bytes_per_cell_device();           // src/lbm.cpp:38-50
+uint bandwidth_bytes_per_cell_device(); // src/lbm.cpp:51-72
+uint3 resolution(const float3 box_aspect_ratio, const uint memory); // grid for a VRAM budget in MB
+fx_lbm_config lbm_config(const uint Nx, const uint Ny, const uint Nz, const float nu); // the compiled switches
+
+class LBM;
+void fx_check(const int status, const string& what); // nonzero status -> print_error, like the reference
+
+// Host buffer + device buffer of one field; the single-domain Memory_Container of the
+// reference (src/lbm.hpp:239-408).  Components are stored one after another (u.x[n] at
+// n, u.y[n] at N+n, u.z[n] at 2N+n).
+template<typename T> class Memory_Container {
+	LBM* lbm = nullptr;
+	ulong N = 0ull;
+	uint d = 1u;
+	string name;
+	vector<T> host;
+	void* device = nullptr;
+	void write_vtk(const string& path, const bool convert_to_si_units);
+public:
+	class Pointer {
+		Memory_Container* m = nullptr;
+		uint dimension = 0u;
+	public:
+		Pointer() {}
+		Pointer(Memory_Container* m, const uint dimension) : m(m), dimension(dimension) {}
+		T& operator[](const ulong i) { return m->host[i + (ulong)dimension * m->N]; }
+		const T& operator[](const ulong i) const { return m->host[i + (ulong)dimension * m->N]; }
+	};
+	Pointer x, y, z;
+
+	Memory_Container() {}
+	Memory_Container(const Memory_Container&) = delete;
+	Memory_Container& operator=(const Memory_Container&) = delete;
+	void allocate(LBM* lbm, const ulong N, const uint d, const string& name, const T fill);
+	void release();
+
+	T& operator[](const ulong i) { return host[i]; }
+	const T& operator[](const ulong i) const { return host[i]; }
+	T operator()(const ulong i) const { return host[i]; }
+	T operator()(const ulong i, const uint dimension) const { return host[i + (ulong)dimension * N]; }
+	ulong length() const { return N; }
+	uint dimensions() const { return d; }
+	ulong range() const { return N * (ulong)d; }
+	ulong capacity() const { return N * (ulong)d * sizeof(T); }
+	T* data() { return host.data(); }
+	void* device_buffer() const { return device; }
+	void reset(const T value = (T)0) { std::fill(host.begin(), host.end(), value); }
+	void read_from_device();  // refreshes rho/u on the device first when stale (src/lbm.hpp:390-396)
+	void write_to_device();
+	void write_host_to_vtk(const string& path = "", const bool convert_to_si_units = true);
+	void write_device_to_vtk(const string& path = "", const bool convert_to_si_units = true);
+};
+
+class LBM {
+	uint Nx = 1u, Ny = 1u, Nz = 1u;
+	uint Dx = 1u, Dy = 1u, Dz = 1u;
+	float nu = 1.0f / 6.0f;
+	float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+	ulong t = 0ull;
+	bool initialized = false;
+	ulong t_last_update_fields = max_ulong;
+	ulong t_last_force_field = max_ulong;
+	fx_ctx* ctx = nullptr;
+	void* fi = nullptr; // DDFs, device only
+	fx_lbm_config cfg{};
+
+	void construct(const uint Nx, const uint Ny, const uint Nz, const uint Dx, const uint Dy, const uint Dz, const float nu, const float fx, const float fy, const float fz,
+	               const float sigma, const float alpha, const float beta, const uint particles_N, const float particles_rho);
+	void sanity_checks_initialization();
+	void initialize();
+	template<typename T> friend class Memory_Container;
+
+public:
+	Memory_Container<float> rho;   // density
+	Memory_Container<float> u;     // velocity
+	Memory_Container<uchar> flags; // cell type bits
+#ifdef FORCE_FIELD
+	Memory_Container<float> F;     // force per cell
+#endif
+#ifdef POROSITY
+	Memory_Container<float> porosity; // Brinkman porosity alpha per cell (not in the reference)
+#endif
+
+	LBM(const uint Nx, const uint Ny, const uint Nz, const uint Dx, const uint Dy, const uint Dz, const float nu, const float fx = 0.0f, const float fy = 0.0f, const float fz = 0.0f, const float sigma = 0.0f, const float alpha = 0.0f, const float beta = 0.0f, const uint particles_N = 0u, const float particles_rho = 0.0f);
+	LBM(const uint Nx, const uint Ny, const uint Nz, const float nu, const float fx = 0.0f, const float fy = 0.0f, const float fz = 0.0f, const float sigma = 0.0f, const float alpha = 0.0f, const float beta = 0.0f, const uint particles_N = 0u, const float particles_rho = 1.0f);
+	LBM(const uint Nx, const uint Ny, const uint Nz, const float nu, const uint particles_N, const float particles_rho = 1.0f);
+	LBM(const uint Nx, const uint Ny, const uint Nz, const float nu, const float fx, const float fy, const float fz, const uint particles_N, const float particles_rho = 1.0f);
+	LBM(const uint3 N, const uint Dx, const uint Dy, const uint Dz, const float nu, const float fx = 0.0f, const float fy = 0.0f, const float fz = 0.0f, const float sigma = 0.0f, const float alpha = 0.0f, const float beta = 0.0f, const uint particles_N = 0u, const float particles_rho = 0.0f);
+	LBM(const uint3 N, const float nu, const float fx = 0.0f, const float fy = 0.0f, const float fz = 0.0f, const float sigma = 0.0f, const float alpha = 0.0f, const float beta = 0.0f, const uint particles_N = 0u, const float particles_rho = 1.0f);
+	LBM(const uint3 N, const float nu, const uint particles_N, const float particles_rho = 1.0f);
+	LBM(const uint3 N, const float nu, const float fx, const float fy, const float fz, const uint particles_N, const float particles_rho = 1.0f);
+	LBM(const LBM&) = delete;
+	LBM& operator=(const LBM&) = delete;
+	~LBM();
+
+	void run(const ulong steps = max_ulong, const ulong total_steps = max_ulong); // initialize on first call, then steps
+	void update_fields(); // rho/u from the DDFs at the current step
+	void reset();         // next run() re-initializes from the host fields
+#ifdef FORCE_FIELD
+	void update_force_field();
+	float3 object_center_of_mass(const uchar flag_marker = TYPE_S);
+	float3 object_force(const uchar flag_marker = TYPE_S);
+	float3 object_torque(const float3& rotation_center, const uchar flag_marker = TYPE_S);
+#endif
+	double total_mass(); // FP64 mass invariant of the DDFs (not in the reference)
+
+	uint get_Nx() const { return Nx; }
+	uint get_Ny() const { return Ny; }
+	uint get_Nz() const { return Nz; }
+	ulong get_N() const { return (ulong)Nx * (ulong)Ny * (ulong)Nz; }
+	uint get_Dx() const { return Dx; }
+	uint get_Dy() const { return Dy; }
+	uint get_Dz() const { return Dz; }
+	uint get_D() const { return Dx * Dy * Dz; }
+	float get_nu() const { return nu; }
+	float get_tau() const { return 3.0f * nu + 0.5f; }
+	float get_w() const { return cfg.w; } // relaxation rate the kernels use
+	float get_Re_max() const { return 0.57735027f * std::sqrt((float)(sq(Nx) + sq(Ny) + sq(Nz))) / nu; }
+	float get_fx() const { return fx; }
+	float get_fy() const { return fy; }
+	float get_fz() const { return fz; }
+	ulong get_t() const { return t; }
+	uint get_velocity_set() const { return cfg.velocity_set; }
+	void set_fx(const float v) { fx = v; }
+	void set_fy(const float v) { fy = v; }
+	void set_fz(const float v) { fz = v; }
+	void set_f(const float x, const float y, const float z) { fx = x; fy = y; fz = z; }
+	fx_ctx* context() const { return ctx; }      // the C ABI context (device, stream, kernels)
+	void* ddf_buffer() const { return fi; }      // device DDF array fi[i*N+n]
+
+	void coordinates(const ulong n, uint& x, uint& y, uint& z) const {
+		const ulong r = n % ((ulong)Nx * (ulong)Ny);
+		x = (uint)(r % (ulong)Nx);
+		y = (uint)(r / (ulong)Nx);
+		z = (uint)(n / ((ulong)Nx * (ulong)Ny));
+	}
+	ulong index(const uint x, const uint y, const uint z) const { return (ulong)x + ((ulong)y + (ulong)z * (ulong)Ny) * (ulong)Nx; }
+	ulong index(const uint3 xyz) const { return index(xyz.x, xyz.y, xyz.z); }
+	float3 position(const uint x, const uint y, const uint z) const { return float3((float)x + 0.5f - 0.5f * (float)Nx, (float)y + 0.5f - 0.5f * (float)Ny, (float)z + 0.5f - 0.5f * (float)Nz); }
+	float3 position(const ulong n) const { uint x, y, z; coordinates(n, x, y, z); return position(x, y, z); }
+	float3 relative_position(const uint x, const uint y, const uint z) const { return float3(((float)x + 0.5f) / (float)Nx - 0.5f, ((float)y + 0.5f) / (float)Ny - 0.5f, ((float)z + 0.5f) / (float)Nz - 0.5f); }
+	float3 relative_position(const ulong n) const { uint x, y, z; coordinates(n, x, y, z); return relative_position(x, y, z); }
+	float3 size() const { return float3((float)Nx, (float)Ny, (float)Nz); }
+	float3 center() const { return float3(0.5f * (float)Nx - 0.5f, 0.5f * (float)Ny - 0.5f, 0.5f * (float)Nz - 0.5f); }
+	uint smallest_side_length() const { return min(min(Nx, Ny), Nz); }
+	uint largest_side_length() const { return max(max(Nx, Ny), Nz); }
+	void write_status(const string& path = "");
+};
+
+template<typename T> void Memory_Container<T>::allocate(LBM* l, const ulong n, const uint dims, const string& field_name, const T fill) {
+	lbm = l; N = n; d = dims; name = field_name;
+	if(N * (ulong)d == 0ull) print_error("Memory size must be larger than 0.");
+	host.assign(N * (ulong)d, fill);
+	fx_check(fx_buf_alloc(lbm->ctx, N * (ulong)d * sizeof(T), &device), "allocating " + name);
+	x = Pointer(this, 0u);
+	if(d > 1u) y = Pointer(this, 1u);
+	if(d > 2u) z = Pointer(this, 2u);
+}
+template<typename T> void Memory_Container<T>::release() {
+	if(device && lbm && lbm->ctx) fx_buf_free(lbm->ctx, device);
+	device = nullptr;
+}
+template<typename T> void Memory_Container<T>::read_from_device() {
+#ifndef UPDATE_FIELDS
+	if(lbm->initialized && lbm->t_last_update_fields != lbm->t) {
+		fx_check(fx_lbm_update_fields(lbm->ctx, lbm->fi, (float*)lbm->rho.device_buffer(), (float*)lbm->u.device_buffer(), (const uint8_t*)lbm->flags.device_buffer(), lbm->t, lbm->fx, lbm->fy, lbm->fz,
+#ifdef FORCE_FIELD
+		                              (const float*)lbm->F.device_buffer(),
+#else
+		                              nullptr,
+#endif
+#ifdef POROSITY
+		                              (const float*)lbm->porosity.device_buffer()),
+#else
+		                              nullptr),
+#endif
+		         "update_fields");
+		lbm->t_last_update_fields = lbm->t;
+	}
+#endif
+	fx_check(fx_buf_read(lbm->ctx, device, 0ull, capacity(), host.data(), 1), "reading " + name);
+}
+template<typename T> void Memory_Container<T>::write_to_device() {
+	fx_check(fx_buf_write(lbm->ctx, device, 0ull, capacity(), host.data(), 1), "writing " + name);
+}
+template<typename T> void Memory_Container<T>::write_host_to_vtk(const string& path, const bool convert_to_si_units) {
+	const string digits = std::to_string(lbm->get_t());
+	const string stamp = string(9u - std::min<size_t>(9u, digits.size()), '0') + digits;
+	write_vtk(path == "" ? name + "-" + stamp + ".vtk" : path, convert_to_si_units);
+}
+template<typename T> void Memory_Container<T>::write_device_to_vtk(const string& path, const bool convert_to_si_units) {
+	read_from_device();
+	write_host_to_vtk(path, convert_to_si_units);
+}
+// Binary legacy VTK STRUCTURED_POINTS, big-endian, components interleaved per point, the
+// format of the reference's write_vtk (src/lbm.hpp:298-340).
+template<typename T> void Memory_Container<T>::write_vtk(const string& path, const bool convert_to_si_units) {
+	const uint nx = lbm->get_Nx(), ny = lbm->get_Ny(), nz = lbm->get_Nz();
+	float spacing = 1.0f, factor = 1.0f;
+	if(convert_to_si_units) {
+		spacing = units.si_x(1.0f);
+		if(name == "rho") factor = units.si_rho(1.0f);
+		if(name == "u") factor = units.si_u(1.0f);
+		if(name == "F") factor = units.si_F(1.0f);
+	}
+	const char* type = std::is_same<T, float>::value ? "float" : std::is_same<T, uchar>::value ? "unsigned_char" : "double";
+	std::ostringstream header;
+	header << "# vtk DataFile Version 3.0\nFluidX3D " << path.substr(path.rfind('/') + 1) << "\nBINARY\nDATASET STRUCTURED_POINTS\n"
+	       << "DIMENSIONS " << nx << " " << ny << " " << nz << "\n"
+	       << "ORIGIN " << to_string(spacing * (0.5f - 0.5f * (float)nx)) << " " << to_string(spacing * (0.5f - 0.5f * (float)ny)) << " " << to_string(spacing * (0.5f - 0.5f * (float)nz)) << "\n"
+	       << "SPACING " << to_string(spacing) << " " << to_string(spacing) << " " << to_string(spacing) << "\n"
+	       << "POINT_DATA " << N << "\nSCALARS data " << type << " " << d << "\nLOOKUP_TABLE default\n";
+	FILE* file = std::fopen(path.c_str(), "wb");
+	if(!file) print_error("cannot open \"" + path + "\" for writing");
+	const string h = header.str();
+	std::fwrite(h.data(), 1, h.size(), file);
+	vector<T> chunk;
+	const ulong per_chunk = 1ull << 20;
+	for(ulong a = 0ull; a < N; a += per_chunk) {
+		const ulong b = std::min(N, a + per_chunk);
+		chunk.resize((b - a) * d);
+		for(ulong n = a; n < b; n++) {
+			for(uint c = 0u; c < d; c++) {
+				T v = host[n + (ulong)c * N];
+				if(!std::is_same<T, uchar>::value) v = (T)(factor * (float)v);
+				uchar* bytes = (uchar*)&v;
+				std::reverse(bytes, bytes + sizeof(T)); // big-endian
+				chunk[(n - a) * d + c] = v;
+			}
+		}
+		std::fwrite(chunk.data(), sizeof(T), chunk.size(), file);
+	}
+	std::fclose(file);
+	print_info("File \"" + path + "\" saved.");
+}

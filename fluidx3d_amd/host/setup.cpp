@@ -1,0 +1,98 @@
+// Scenes written against the reference's LBM API.  Select one with -DSCENE_<NAME> (the
+// reference selects by commenting blocks in src/setup.cpp); BENCHMARK selects the benchmark.
+#include "setup.hpp"
+
+#if defined(BENCHMARK)
+// Peak MLUPs/s of an empty periodic 256^3 box at nu=1 over 1000 x run(10)
+// (same scene as the reference's benchmark, src/setup.cpp:5-36).
+void main_setup() {
+	uint mlups = 0u;
+	{
+		LBM lbm(256u, 256u, 256u, 1.0f);
+		for(uint i = 0u; i < 1000u; i++) {
+			lbm.run(10u, 1000u * 10u);
+			mlups = max(mlups, to_uint((double)lbm.get_N() * 1E-6 / info.runtime_lbm_timestep_smooth));
+		}
+	}
+	print_info("Peak MLUPs/s = " + to_string(mlups));
+}
+
+#elif defined(SCENE_TAYLOR_GREEN)
+#ifndef TGV_N
+#define TGV_N 64u
+#endif
+#ifndef TGV_STEPS
+#define TGV_STEPS 1000u
+#endif
+// 3D Taylor-Green vortex, u0 = 0.05, nu = 1/6, periodic (workload of the 64^3 and 256^3
+// configurations); prints the centre-cell u.x like the reference's binary-identity line
+// (src/setup.cpp:58) and writes u as VTK when FX_VTK is set.
+void main_setup() {
+	LBM lbm(TGV_N, TGV_N, TGV_N, 1.0f / 6.0f);
+	const uint Nx = lbm.get_Nx(), Ny = lbm.get_Ny(), Nz = lbm.get_Nz();
+	const double u0 = 0.05; // fields evaluated in double on cell centres, then rounded to float
+	parallel_for(lbm.get_N(), [&](ulong n) {
+		uint x = 0u, y = 0u, z = 0u; lbm.coordinates(n, x, y, z);
+		const double px = (double)x + 0.5 - 0.5 * (double)Nx, py = (double)y + 0.5 - 0.5 * (double)Ny, pz = (double)z + 0.5 - 0.5 * (double)Nz;
+		const double kx = 2.0 * pi / (double)Nx, ky = 2.0 * pi / (double)Ny, kz = 2.0 * pi / (double)Nz;
+		lbm.u.x[n] = (float)(u0 * std::sin(kx * px) * std::cos(ky * py) * std::cos(kz * pz));
+		lbm.u.y[n] = (float)(-u0 * std::cos(kx * px) * std::sin(ky * py) * std::cos(kz * pz));
+		lbm.u.z[n] = 0.0f;
+		lbm.rho[n] = (float)(1.0 + 3.0 * (u0 * u0 / 16.0) * (std::cos(2.0 * kx * px) + std::cos(2.0 * ky * py)) * (std::cos(2.0 * kz * pz) + 2.0));
+	});
+	const double m0 = (lbm.run(0u), lbm.total_mass());
+	lbm.run(TGV_STEPS);
+	lbm.u.read_from_device();
+	lbm.rho.read_from_device();
+	double ke = 0.0;
+	for(ulong n = 0ull; n < lbm.get_N(); n++) ke += 0.5 * (double)lbm.rho[n] * ((double)sq(lbm.u.x[n]) + (double)sq(lbm.u.y[n]) + (double)sq(lbm.u.z[n]));
+	char line[256];
+	std::snprintf(line, sizeof(line), "t=%llu centre u.x=%.9e kinetic energy=%.12e mass drift=%.3e", (unsigned long long)lbm.get_t(), (double)lbm.u.x[lbm.index(Nx / 2u, Ny / 2u, Nz / 2u)], ke, (lbm.total_mass() - m0) / m0);
+	println(line);
+#ifdef FX_VTK
+	lbm.u.write_host_to_vtk("tgv_u.vtk", false);
+#endif
+}
+
+#elif defined(SCENE_SPHERE)
+#ifndef SPHERE_STEPS
+#define SPHERE_STEPS 1000u
+#endif
+#ifndef SPHERE_SCALE
+#define SPHERE_SCALE 1u
+#endif
+// Flow past a sphere: 512x256x256 (times SPHERE_SCALE), radius 32*scale at
+// (128,128,128)*scale, TYPE_E inflow/outflow u=(0.075,0,0) at x=0 and x=Nx-1, periodic in y
+// and z, Re=100 on the diameter, seeded 1e-4 perturbation of u; drag from the FP64
+// object force sum.  Needs EQUILIBRIUM_BOUNDARIES and FORCE_FIELD.
+void main_setup() {
+	const uint s = SPHERE_SCALE;
+	const float R = 32.0f * (float)s, u_in = 0.075f, Re = 100.0f;
+	const float nu = units.nu_from_Re(Re, 2.0f * R, u_in);
+	LBM lbm(512u * s, 256u * s, 256u * s, nu);
+	const uint Nx = lbm.get_Nx();
+	const float3 c = float3(128.0f * (float)s, 128.0f * (float)s, 128.0f * (float)s);
+	uint seed = 42u;
+	for(ulong n = 0ull; n < lbm.get_N(); n++) { // sequential: the LCG perturbation is seeded and ordered
+		uint x = 0u, y = 0u, z = 0u; lbm.coordinates(n, x, y, z);
+		const float px = random_symmetric(seed, 1E-4f), py = random_symmetric(seed, 1E-4f), pz = random_symmetric(seed, 1E-4f);
+		if(sphere(x, y, z, c, R)) { lbm.flags[n] = TYPE_S | TYPE_X; continue; }
+		lbm.u.x[n] = u_in + px; lbm.u.y[n] = py; lbm.u.z[n] = pz;
+		if(x == 0u || x == Nx - 1u) { lbm.flags[n] = TYPE_E; lbm.u.x[n] = u_in; lbm.u.y[n] = 0.0f; lbm.u.z[n] = 0.0f; }
+	}
+	Clock clock;
+	lbm.run(SPHERE_STEPS);
+#ifdef FORCE_FIELD
+	const float3 F = lbm.object_force(TYPE_S | TYPE_X);
+	const double Cd = (double)F.x / (0.5 * (double)sq(u_in) * pi * (double)sq(R));
+	char line[256];
+	std::snprintf(line, sizeof(line), "t=%llu drag F=(%.9e, %.9e, %.9e) Cd=%.6f MLUPs/s=%.0f", (unsigned long long)lbm.get_t(), (double)F.x, (double)F.y, (double)F.z, Cd, (double)lbm.get_N() * (double)lbm.get_t() * 1E-6 / info.runtime_lbm);
+	println(line);
+#endif
+}
+
+#else
+void main_setup() {
+	print_error("no scene selected: compile with -DBENCHMARK or -DSCENE_TAYLOR_GREEN or -DSCENE_SPHERE");
+}
+#endif

@@ -91,7 +91,7 @@ class LBM:
         self.u = Field(self, "u", 3, np.float32)
         self.flags = Field(self, "flags", 1, np.uint8)
         self.F = Field(self, "F", 3, np.float32) if ext & _abi.FX_FORCE_FIELD else None
-        self.alpha = Field(self, "alpha", 1, np.float32) if ext & _abi.FX_POROSITY else None
+        self.porosity = Field(self, "porosity", 1, np.float32) if ext & _abi.FX_POROSITY else None
         self.t = 0
         self.t_last_update_fields = None
         self.t_last_force_field = None
@@ -130,8 +130,8 @@ class LBM:
         self.flags.write_to_device()
         if self.F is not None:
             self.F.write_to_device()
-        if self.alpha is not None:
-            self.alpha.write_to_device()
+        if self.porosity is not None:
+            self.porosity.write_to_device()
         check(self.ctx.lib.fx_lbm_initialize(self.ctx.h, self.fi, self.rho.device, self.u.device, self.flags.device))
         self.ctx.sync()
         self.t = 0
@@ -149,7 +149,7 @@ class LBM:
             self.initialize()
         for _ in range(int(steps)):
             check(self.ctx.lib.fx_lbm_stream_collide(*self._step_args(), self.t, self.fx, self.fy, self.fz,
-                                                     self._ptr(self.F), self._ptr(self.alpha)))
+                                                     self._ptr(self.F), self._ptr(self.porosity)))
             if sync_each_step:
                 self.ctx.sync()
             self.t += 1
@@ -160,7 +160,7 @@ class LBM:
     def _update_fields_if_stale(self):
         if self.t_last_update_fields != self.t:
             check(self.ctx.lib.fx_lbm_update_fields(*self._step_args(), self.t, self.fx, self.fy, self.fz,
-                                                    self._ptr(self.F), self._ptr(self.alpha)))
+                                                    self._ptr(self.F), self._ptr(self.porosity)))
             self.t_last_update_fields = self.t
 
     def update_fields(self):

@@ -52,7 +52,7 @@ def setup_pair(oracle, Nx, Ny, Nz, nu, q, collision, storage, ext_names, rho, u,
         lbm.F.host[:] = F
         ref.F[:] = F
     if alpha is not None:
-        lbm.alpha.host[:] = alpha
+        lbm.porosity.host[:] = alpha
         ref.alpha[:] = alpha
     return lbm, ref
 

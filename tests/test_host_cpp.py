@@ -1,0 +1,94 @@
+"""The C++ host layer (fluidx3d_amd/host): scenes written against the reference's LBM API,
+built with g++ against libfx_lbm.so."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "fluidx3d_amd", "host", "bin")
+
+
+def binary(name):
+    path = os.path.join(BIN, name)
+    if not os.path.exists(path):
+        pytest.skip(f"{path} not built (make -C fluidx3d_amd/host)")
+    return path
+
+
+def read_vtk(path):
+    """Binary legacy VTK STRUCTURED_POINTS written by Memory_Container::write_vtk."""
+    raw = open(path, "rb").read()
+    head_end = raw.index(b"LOOKUP_TABLE default\n") + len(b"LOOKUP_TABLE default\n")
+    header = raw[:head_end].decode()
+    dims = [int(v) for v in re.search(r"DIMENSIONS (\d+) (\d+) (\d+)", header).groups()]
+    comps = int(re.search(r"SCALARS data float (\d+)", header).group(1))
+    data = np.frombuffer(raw[head_end:], dtype=">f4").astype(np.float32)
+    return dims, data.reshape(-1, comps)
+
+
+def test_scene_without_gpu_fails_like_reference(tmp_path):
+    """No device: the scene prints an error and exits with status 1 (print_error, src/utilities.hpp:4074-4086)."""
+    import fluidx3d_amd as fx
+    try:
+        if fx.device_count() > 0:
+            pytest.skip("a GPU is present")
+    except Exception:
+        pass
+    r = subprocess.run([binary("tgv64")], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1
+    assert "Error" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cpp_taylor_green_matches_python_path(tmp_path):
+    """64^3 TGV, 1000 steps through the C++ LBM class == the same run through the Python
+    mirror (both call libfx_lbm.so).  Both evaluate the initial fields in double and round
+    to float; libm and numpy sin/cos may differ in the last double bit, which can flip a
+    float rounding, so agreement is required to 1e-7 u0 rather than bitwise."""
+    import fluidx3d_amd as fx
+    from tests import scenes
+    r = subprocess.run([binary("tgv64")], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"mass drift=([-0-9.e+]+)", r.stdout)
+    assert m and abs(float(m.group(1))) < 1e-6
+    dims, u_cpp = read_vtk(os.path.join(tmp_path, "tgv_u.vtk"))
+    assert dims == [64, 64, 64]
+    N = 64 ** 3
+    lbm = fx.LBM(64, 64, 64, 1.0 / 6.0)
+    rho, u, flags = scenes.taylor_green(64, 64, 64)
+    lbm.rho.host[:] = rho
+    lbm.u.host[:] = u
+    lbm.run(1000)
+    lbm.u.read_from_device()
+    u_py = lbm.u.host.reshape(3, N).T
+    lbm.close()
+    assert np.max(np.abs(u_cpp - u_py)) < 5e-9
+    exact = np.mean(u_cpp == u_py)
+    print(f"\nC++ vs Python u: {100 * exact:.4f}% bit-identical, max |diff| {np.max(np.abs(u_cpp - u_py)):.2e}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["benchmark_fp32", "benchmark_fp16s", "benchmark_fp16c"])
+def test_cpp_benchmark_scene(tmp_path, name):
+    """The reference's benchmark scene (256^3, nu=1, 1000 x run(10), peak smoothed MLUPs/s)."""
+    r = subprocess.run([binary(name)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    mlups = int(re.search(r"Peak MLUPs/s = (\d+)", r.stdout).group(1))
+    print(f"\n{name}: peak {mlups} MLUPs/s")
+    floor = {"benchmark_fp32": 30380, "benchmark_fp16s": 54494, "benchmark_fp16c": 40157}[name]  # reference, README.md:1137
+    assert mlups > floor
+
+
+@pytest.mark.gpu
+def test_cpp_sphere_scene(tmp_path):
+    """D3Q27 TRT FP16C sphere with TYPE_E inflow/outflow: 1000 steps, finite positive drag."""
+    r = subprocess.run([binary("sphere")], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"drag F=\(([-0-9.e+]+), ([-0-9.e+]+), ([-0-9.e+]+)\) Cd=([-0-9.e+]+)", r.stdout)
+    assert m, r.stdout
+    Fx, Fy, Fz, Cd = (float(v) for v in m.groups())
+    print("\n" + r.stdout.strip().splitlines()[-1])
+    assert np.isfinite([Fx, Fy, Fz, Cd]).all() and Fx > 0 and abs(Fy) < 0.1 * Fx and abs(Fz) < 0.1 * Fx
