@@ -38,6 +38,7 @@ def parse_args():
     p.add_argument("--sync-steps", type=int, default=200, help="steps of the per-step-synchronised figure")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-adjoint", action="store_true")
     return p.parse_args()
 
 
@@ -102,6 +103,37 @@ def gpu_mode(fx, size, storage, steps, warmup, sync_steps):
     }
 
 
+def adjoint_timing(fx, dims, steps, window, interval):
+    """Checkpointed adjoint of the porous-channel objective: GPU time of fx_lbm_forward
+    (checkpoints + objective) and fx_lbm_backward (recompute + adjoint) per step, against
+    the plain forward step (stream_collide with the same extensions) on the same grid."""
+    import ctypes
+    from fluidx3d_amd import _abi
+    from fluidx3d_amd.adjoint import Adjoint
+    from tests import scenes
+    rho, u, flags, alpha = scenes.channel_c4(*dims)
+    lbm = fx.LBM(*dims, 1.0 / 6.0, 1e-6, 0.0, 0.0, extensions=("VOLUME_FORCE", "POROSITY", "PRECISE_DENSITY"))
+    lbm.rho.host[:] = rho; lbm.u.host[:] = u; lbm.flags.host[:] = flags; lbm.porosity.host[:] = alpha
+    lbm.initialize()
+    ms = ctypes.c_float()
+    args = (lbm.ctx.h, lbm.fi, lbm.rho.device, lbm.u.device, lbm.flags.device)
+    plain_steps = min(steps, 500)
+    _abi.check(lbm.ctx.lib.fx_lbm_run(*args, 0, 20, lbm.fx, 0.0, 0.0, None, lbm.porosity.device, ctypes.byref(ms)))
+    _abi.check(lbm.ctx.lib.fx_lbm_run(*args, 20, plain_steps, lbm.fx, 0.0, 0.0, None, lbm.porosity.device, ctypes.byref(ms)))
+    plain = ms.value / plain_steps
+    adj = Adjoint(lbm, steps, window, dims[0] - 1, interval)
+    mem = Adjoint.memory_bytes(lbm, steps, window, dims[0] - 1, interval)
+    J = adj.forward()
+    grad = adj.backward()
+    fwd, bwd = adj.forward_ms / steps, adj.backward_ms / steps
+    adj.close()
+    lbm.close()
+    return {"workload": f"{dims[0]}x{dims[1]}x{dims[2]} porous channel, {steps} steps, checkpoint every {adj.cfg.checkpoint_interval or 'ceil(sqrt(T))'} steps",
+            "plain_forward_ms_per_step": round(plain, 5), "forward_ms_per_step": round(fwd, 5), "backward_ms_per_step": round(bwd, 5),
+            "backward_over_plain_forward": round(bwd / plain, 3), "forward_plus_backward_over_plain": round((fwd + bwd) / plain, 3),
+            "workspace_GB": round(mem / 1e9, 2), "objective": J, "grad_finite": bool(np.isfinite(grad).all())}
+
+
 def measured_copy_peak(fx):
     from fluidx3d_amd import _abi
     import ctypes
@@ -157,6 +189,12 @@ def main():
         print(f"# {m}: {results[m]['mlups']:.0f} MLUPs/s, {results[m]['achieved_GBps']:.0f} GB/s "
               f"({100 * results[m]['frac']:.1f}% of 8 TB/s)", file=sys.stderr, flush=True)
     copy_gbps = measured_copy_peak(fx)
+    adjoint = None
+    if not a.no_adjoint:
+        adjoint = {"scaled": adjoint_timing(fx, (384, 192, 192), 5000, 3750, 0),
+                   "channel_128x64x64": adjoint_timing(fx, (128, 64, 64), 2000, 1500, 100)}
+        print(f"# adjoint: backward/plain-forward {adjoint['scaled']['backward_over_plain_forward']}x at 384x192x192",
+              file=sys.stderr, flush=True)
     primary = "FP32" if "FP32" in results else modes[0]
     r = results[primary]
     traffic = read_traffic(primary)
@@ -182,6 +220,7 @@ def main():
         "frac_of_measured_copy": round(r["achieved_GBps"] / copy_gbps, 4),
         "modes": {m: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in results[m].items()} for m in results},
         "reference_mlups_mi355x": REFERENCE_MLUPS,
+        "adjoint": adjoint,
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(min(a.size, 256), a.cpu_seconds)

@@ -18,12 +18,13 @@ FX_FORCE_FIELD = 2
 FX_EQUILIBRIUM_BOUNDARIES = 4
 FX_UPDATE_FIELDS = 8
 FX_POROSITY = 16
+FX_PRECISE_DENSITY = 32
 
 STORAGE = {"FP32": FX_FP32, "FP16S": FX_FP16S, "FP16C": FX_FP16C}
 COLLISION = {"SRT": FX_SRT, "TRT": FX_TRT}
 EXTENSIONS = {"VOLUME_FORCE": FX_VOLUME_FORCE, "FORCE_FIELD": FX_FORCE_FIELD,
               "EQUILIBRIUM_BOUNDARIES": FX_EQUILIBRIUM_BOUNDARIES, "UPDATE_FIELDS": FX_UPDATE_FIELDS,
-              "POROSITY": FX_POROSITY}
+              "POROSITY": FX_POROSITY, "PRECISE_DENSITY": FX_PRECISE_DENSITY}
 
 # every symbol include/fx_lbm.h declares
 EXPORTS = [
@@ -32,7 +33,9 @@ EXPORTS = [
     "fx_buf_free", "fx_buf_write", "fx_buf_read", "fx_buf_fill", "fx_buf_copy", "fx_lbm_initialize",
     "fx_lbm_stream_collide", "fx_lbm_run", "fx_lbm_update_fields", "fx_lbm_update_force_field",
     "fx_lbm_reset_force_field", "fx_lbm_object_force", "fx_lbm_object_torque", "fx_lbm_object_center_of_mass",
-    "fx_lbm_mass_sum", "fx_codec_roundtrip", "fx_bench_copy",
+    "fx_lbm_mass_sum", "fx_codec_roundtrip", "fx_bench_copy", "fx_adjoint_memory", "fx_adjoint_create",
+    "fx_adjoint_destroy", "fx_lbm_forward", "fx_lbm_backward", "fx_lbm_stream_collide_record", "fx_lbm_adjoint_step",
+    "fx_lbm_objective_plane",
 ]
 
 
@@ -40,6 +43,11 @@ class Config(ctypes.Structure):
     _fields_ = [("Nx", ctypes.c_uint32), ("Ny", ctypes.c_uint32), ("Nz", ctypes.c_uint32),
                 ("velocity_set", ctypes.c_uint32), ("collision", ctypes.c_uint32), ("storage", ctypes.c_uint32),
                 ("extensions", ctypes.c_uint32), ("w", ctypes.c_float)]
+
+
+class AdjointConfig(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint64), ("checkpoint_interval", ctypes.c_uint64), ("window_begin", ctypes.c_uint64),
+                ("plane_x", ctypes.c_uint32)]
 
 
 class DeviceInfo(ctypes.Structure):
@@ -97,6 +105,16 @@ def load():
     L.fx_lbm_object_center_of_mass.argtypes = [vp, vp, ctypes.c_uint8, dp]
     L.fx_lbm_mass_sum.argtypes = [vp, vp, vp, u64, dp]
     L.fx_codec_roundtrip.argtypes = [vp, u32, vp, vp, vp, u64]
+    acp = ctypes.POINTER(AdjointConfig)
+    fp = ctypes.POINTER(f32)
+    L.fx_adjoint_memory.argtypes = [vp, acp, ctypes.POINTER(u64)]
+    L.fx_adjoint_create.argtypes = [vp, acp, ctypes.POINTER(vp)]
+    L.fx_adjoint_destroy.argtypes = [vp]
+    L.fx_lbm_forward.argtypes = [vp, vp, vp, vp, vp, vp, f32, f32, f32, vp, vp, dp, fp]
+    L.fx_lbm_backward.argtypes = [vp, vp, vp, vp, vp, f32, f32, f32, vp, vp, vp, fp]
+    L.fx_lbm_stream_collide_record.argtypes = [vp, vp, vp, vp, vp, u64, f32, f32, f32, vp, vp, vp]
+    L.fx_lbm_adjoint_step.argtypes = [vp, vp, vp, vp, vp, u64, f32, f32, f32, vp, vp, vp, u32, f32, vp]
+    L.fx_lbm_objective_plane.argtypes = [vp, vp, vp, u64, u32, dp]
     L.fx_bench_copy.argtypes = [vp, vp, vp, u64, u32, ctypes.POINTER(f32)]
     _lib = L
     return L

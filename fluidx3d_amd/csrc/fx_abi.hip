@@ -27,7 +27,7 @@ int fail_hip(hipError_t e, const char* what) {
 
 bool valid_enums(const fx_lbm_config* c) {
 	return c && (c->velocity_set == FX_D3Q19 || c->velocity_set == FX_D3Q27) && c->collision <= FX_TRT && c->storage <= FX_FP16C &&
-	       (c->extensions & ~(FX_VOLUME_FORCE | FX_FORCE_FIELD | FX_EQUILIBRIUM_BOUNDARIES | FX_UPDATE_FIELDS | FX_POROSITY)) == 0u;
+	       (c->extensions & ~(FX_VOLUME_FORCE | FX_FORCE_FIELD | FX_EQUILIBRIUM_BOUNDARIES | FX_UPDATE_FIELDS | FX_POROSITY | FX_PRECISE_DENSITY)) == 0u;
 }
 
 } // namespace
@@ -44,6 +44,19 @@ struct fx_ctx {
 	uint64_t partial_cap = 0;  // in doubles
 	double* result = nullptr;  // 4 doubles (device)
 	std::set<void*> buffers;   // allocations owned by this context
+};
+
+struct fx_adjoint {
+	fx_ctx* ctx = nullptr;
+	fx_adjoint_config cfg{};
+	uint64_t k = 0, checkpoints = 0, fi_bytes = 0, obj_blocks = 0;
+	char* ckpt = nullptr;       // checkpoints * fi_bytes
+	void* work_fi = nullptr;    // recomputation state
+	float* moments = nullptr;   // k * 4 * N
+	float* lambda = nullptr;    // q * N
+	double* partial = nullptr;  // obj_blocks
+	double* acc = nullptr;      // 1
+	bool forward_done = false;
 };
 
 namespace {
@@ -196,6 +209,20 @@ __global__ __launch_bounds__(RED_BLOCK) void k_reduce_partials(const double* par
 		if(threadIdx.x == 0) out[k] = red[0];
 		__syncthreads();
 	}
+}
+
+// acc[0] += sum of count partials (single block, fixed order)
+__global__ __launch_bounds__(RED_BLOCK) void k_reduce_add(const double* partial, uint64_t count, double* acc) {
+	__shared__ double red[RED_BLOCK];
+	double s = 0.0;
+	for(uint64_t b = threadIdx.x; b < count; b += RED_BLOCK) s += partial[b];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for(int h = RED_BLOCK / 2; h > 0; h >>= 1) {
+		if((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+		__syncthreads();
+	}
+	if(threadIdx.x == 0) acc[0] += red[0];
 }
 
 __global__ void k_reset3(float* F, uint64_t n3) {
@@ -509,6 +536,196 @@ int fx_lbm_mass_sum(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t 
 	if(rc != FX_OK) return rc;
 	FX_HIP(hipMemcpyAsync(out, ctx->result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
 	FX_HIP(hipStreamSynchronize(ctx->stream));
+	return FX_OK;
+}
+
+static uint64_t ddf_bytes(const fx_ctx* ctx) {
+	return ctx->N * (uint64_t)ctx->cfg.velocity_set * (ctx->cfg.storage == FX_FP32 ? 4u : 2u);
+}
+static int adjoint_layout(const fx_ctx* ctx, const fx_adjoint_config* cfg, uint64_t& k, uint64_t& checkpoints, uint64_t& bytes) {
+	if(!ctx || !cfg) return fail(FX_ERR_ARG, "null argument");
+	if(cfg->steps == 0u || cfg->window_begin >= cfg->steps) return fail(FX_ERR_ARG, "need 0 <= window_begin < steps");
+	if(cfg->plane_x >= ctx->cfg.Nx) return fail(FX_ERR_ARG, "plane_x out of range");
+	k = cfg->checkpoint_interval;
+	if(k == 0u) { k = (uint64_t)std::ceil(std::sqrt((double)cfg->steps)); if(k == 0u) k = 1u; }
+	if(k > cfg->steps) k = cfg->steps;
+	checkpoints = (cfg->steps + k - 1u) / k;
+	const uint64_t N = ctx->N, q = ctx->cfg.velocity_set;
+	const uint64_t obj_blocks = ((uint64_t)ctx->cfg.Ny * ctx->cfg.Nz + 255u) / 256u;
+	bytes = checkpoints * ddf_bytes(ctx) + ddf_bytes(ctx) + k * 4u * N * 4u + q * N * 4u + (obj_blocks + 1u) * 8u;
+	return FX_OK;
+}
+
+int fx_adjoint_memory(const fx_ctx* ctx, const fx_adjoint_config* cfg, uint64_t* bytes) {
+	uint64_t k, c, b;
+	const int rc = adjoint_layout(ctx, cfg, k, c, b);
+	if(rc != FX_OK) return rc;
+	if(!bytes) return fail(FX_ERR_ARG, "null output");
+	*bytes = b;
+	return FX_OK;
+}
+
+int fx_adjoint_create(fx_ctx* ctx, const fx_adjoint_config* cfg, fx_adjoint** out) {
+	if(!out) return fail(FX_ERR_ARG, "null output handle");
+	*out = nullptr;
+	uint64_t k, checkpoints, bytes;
+	int rc = adjoint_layout(ctx, cfg, k, checkpoints, bytes);
+	if(rc != FX_OK) return rc;
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	size_t free_b = 0, total_b = 0;
+	FX_HIP(hipMemGetInfo(&free_b, &total_b));
+	if(bytes > (uint64_t)free_b) return fail(FX_ERR_MEMORY, "adjoint workspace of " + std::to_string(bytes) + " bytes does not fit the free device memory");
+	fx_adjoint* adj = new fx_adjoint();
+	adj->ctx = ctx; adj->cfg = *cfg; adj->k = k; adj->checkpoints = checkpoints; adj->fi_bytes = ddf_bytes(ctx);
+	adj->obj_blocks = ((uint64_t)ctx->cfg.Ny * ctx->cfg.Nz + 255u) / 256u;
+	hipError_t e = hipMalloc(&adj->ckpt, checkpoints * adj->fi_bytes);
+	if(e == hipSuccess) e = hipMalloc(&adj->work_fi, adj->fi_bytes);
+	if(e == hipSuccess) e = hipMalloc(&adj->moments, k * 4u * ctx->N * sizeof(float));
+	if(e == hipSuccess) e = hipMalloc(&adj->lambda, (uint64_t)ctx->cfg.velocity_set * ctx->N * sizeof(float));
+	if(e == hipSuccess) e = hipMalloc(&adj->partial, adj->obj_blocks * sizeof(double));
+	if(e == hipSuccess) e = hipMalloc(&adj->acc, sizeof(double));
+	if(e != hipSuccess) { fx_adjoint_destroy(adj); return fail(FX_ERR_MEMORY, std::string("adjoint workspace: ") + hipGetErrorString(e)); }
+	*out = adj;
+	return FX_OK;
+}
+
+int fx_adjoint_destroy(fx_adjoint* adj) {
+	if(!adj) return FX_OK;
+	(void)hipSetDevice(adj->ctx->device);
+	(void)hipStreamSynchronize(adj->ctx->stream);
+	if(adj->ckpt) (void)hipFree(adj->ckpt);
+	if(adj->work_fi) (void)hipFree(adj->work_fi);
+	if(adj->moments) (void)hipFree(adj->moments);
+	if(adj->lambda) (void)hipFree(adj->lambda);
+	if(adj->partial) (void)hipFree(adj->partial);
+	if(adj->acc) (void)hipFree(adj->acc);
+	delete adj;
+	return FX_OK;
+}
+
+int fx_lbm_stream_collide_record(fx_ctx* ctx, void* fi, float* rho, float* u, const uint8_t* flags, uint64_t t, float fx_, float fy_, float fz_,
+                                 const float* F, const float* alpha, float* moments) {
+	const int rc = check_step_args(ctx, fi, rho, u, flags, F, alpha);
+	if(rc != FX_OK) return rc;
+	if(!moments) return fail(FX_ERR_ARG, "null moments");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	ctx->k->stream_collide_record(make_args(ctx, fi, rho, u, flags, t, fx_, fy_, fz_, F, alpha), ctx->shape, ctx->stream, moments);
+	return check_launch("stream_collide_record");
+}
+
+static fx::AdjointArgs adjoint_args(const fx_ctx* ctx, float* lambda, const float* rho, const float* u, const uint8_t* flags, uint64_t t, float fx_, float fy_, float fz_,
+                                    const float* F, const float* alpha, const float* moments, uint32_t plane_x, float obj_weight, float* grad) {
+	fx::AdjointArgs g;
+	g.a = make_args(ctx, lambda, rho, u, flags, t, fx_, fy_, fz_, F, alpha);
+	g.moments = moments;
+	g.grad = grad;
+	g.plane_x = plane_x;
+	g.obj_weight = obj_weight;
+	return g;
+}
+
+int fx_lbm_adjoint_step(fx_ctx* ctx, float* lambda, const float* rho, const float* u, const uint8_t* flags, uint64_t t, float fx_, float fy_, float fz_,
+                        const float* F, const float* alpha, const float* moments, uint32_t plane_x, float obj_weight, float* grad_alpha) {
+	int rc = check_step_args(ctx, lambda, rho, u, flags, F, alpha);
+	if(rc != FX_OK) return rc;
+	if(!moments) return fail(FX_ERR_ARG, "null moments");
+	if((ctx->cfg.extensions & FX_POROSITY) && !grad_alpha) return fail(FX_ERR_ARG, "POROSITY needs grad_alpha");
+	if((ctx->cfg.extensions & FX_EQUILIBRIUM_BOUNDARIES) && (!rho || !u)) return fail(FX_ERR_ARG, "TYPE_E presets rho/u required");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	ctx->k->adjoint_step(adjoint_args(ctx, lambda, rho, u, flags, t, fx_, fy_, fz_, F, alpha, moments, plane_x, obj_weight, grad_alpha), ctx->shape, ctx->stream);
+	return check_launch("adjoint_step");
+}
+
+static int objective_accumulate(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t t, uint32_t plane_x, double* partial, uint64_t blocks, double* acc) {
+	ctx->k->objective_partial(make_args(ctx, fi, nullptr, nullptr, flags, t, 0.0f, 0.0f, 0.0f, nullptr, nullptr), plane_x, ctx->stream, partial);
+	fx::k_reduce_add<<<1, fx::RED_BLOCK, 0, ctx->stream>>>(partial, blocks, acc);
+	return check_launch("objective");
+}
+
+int fx_lbm_objective_plane(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t t, uint32_t plane_x, double* out) {
+	if(!ctx || !fi || !flags || !out) return fail(FX_ERR_ARG, "null argument");
+	if(plane_x >= ctx->cfg.Nx) return fail(FX_ERR_ARG, "plane_x out of range");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	const uint64_t blocks = ((uint64_t)ctx->cfg.Ny * ctx->cfg.Nz + 255u) / 256u;
+	int rc = ensure_partial(ctx, blocks);
+	if(rc != FX_OK) return rc;
+	FX_HIP(hipMemsetAsync(ctx->result, 0, sizeof(double), ctx->stream));
+	rc = objective_accumulate(ctx, fi, flags, t, plane_x, ctx->partial, blocks, ctx->result);
+	if(rc != FX_OK) return rc;
+	FX_HIP(hipMemcpyAsync(out, ctx->result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+	FX_HIP(hipStreamSynchronize(ctx->stream));
+	return FX_OK;
+}
+
+int fx_lbm_forward(fx_ctx* ctx, fx_adjoint* adj, void* fi, float* rho, float* u, const uint8_t* flags, float fx_, float fy_, float fz_,
+                   const float* F, const float* alpha, double* objective, float* ms) {
+	int rc = check_step_args(ctx, fi, rho, u, flags, F, alpha);
+	if(rc != FX_OK) return rc;
+	if(!adj || adj->ctx != ctx) return fail(FX_ERR_ARG, "adjoint handle does not belong to this context");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	const fx_adjoint_config& c = adj->cfg;
+	FX_HIP(hipMemsetAsync(adj->acc, 0, sizeof(double), ctx->stream));
+	FX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+	fx::LbmArgs a = make_args(ctx, fi, rho, u, flags, 0u, fx_, fy_, fz_, F, alpha);
+	for(uint64_t t = 0; t < c.steps; t++) {
+		if(t % adj->k == 0u) FX_HIP(hipMemcpyAsync(adj->ckpt + (t / adj->k) * adj->fi_bytes, fi, adj->fi_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+		if(t >= c.window_begin) {
+			rc = objective_accumulate(ctx, fi, flags, t, c.plane_x, adj->partial, adj->obj_blocks, adj->acc);
+			if(rc != FX_OK) return rc;
+		}
+		a.odd = (uint32_t)(t & 1u);
+		ctx->k->stream_collide(a, ctx->shape, ctx->stream);
+	}
+	rc = check_launch("forward");
+	if(rc != FX_OK) return rc;
+	FX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+	double sum = 0.0;
+	FX_HIP(hipMemcpyAsync(&sum, adj->acc, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+	FX_HIP(hipStreamSynchronize(ctx->stream));
+	if(ms) FX_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+	if(objective) *objective = sum / (double)(c.steps - c.window_begin);
+	adj->forward_done = true;
+	return FX_OK;
+}
+
+int fx_lbm_backward(fx_ctx* ctx, fx_adjoint* adj, const float* rho, const float* u, const uint8_t* flags, float fx_, float fy_, float fz_,
+                    const float* F, const float* alpha, float* grad_alpha, float* ms) {
+	if(!adj || adj->ctx != ctx) return fail(FX_ERR_ARG, "adjoint handle does not belong to this context");
+	if(!adj->forward_done) return fail(FX_ERR_STATE, "fx_lbm_forward must run before fx_lbm_backward");
+	int rc = check_step_args(ctx, adj->work_fi, rho, u, flags, F, alpha);
+	if(rc != FX_OK) return rc;
+	if((ctx->cfg.extensions & FX_POROSITY) && !grad_alpha) return fail(FX_ERR_ARG, "POROSITY needs grad_alpha");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	const fx_adjoint_config& c = adj->cfg;
+	const uint64_t N = ctx->N;
+	// rho/u are only read (TYPE_E presets); UPDATE_FIELDS writes of the recomputation go to scratch
+	float* rho_w = const_cast<float*>(rho);
+	float* u_w = const_cast<float*>(u);
+	FX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+	FX_HIP(hipMemsetAsync(adj->lambda, 0, (uint64_t)ctx->cfg.velocity_set * N * sizeof(float), ctx->stream));
+	if(grad_alpha) FX_HIP(hipMemsetAsync(grad_alpha, 0, N * sizeof(float), ctx->stream));
+	const float wobj = (float)(1.0 / (double)(c.steps - c.window_begin));
+	fx::LbmArgs a = make_args(ctx, adj->work_fi, rho_w, u_w, flags, 0u, fx_, fy_, fz_, F, alpha);
+	a.ext &= ~FX_UPDATE_FIELDS; // the recomputation must not overwrite rho/u
+	for(uint64_t s = adj->checkpoints; s-- > 0u;) {
+		const uint64_t ts = s * adj->k, te = std::min<uint64_t>(c.steps, ts + adj->k);
+		FX_HIP(hipMemcpyAsync(adj->work_fi, adj->ckpt + s * adj->fi_bytes, adj->fi_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+		for(uint64_t t = ts; t < te; t++) {
+			a.odd = (uint32_t)(t & 1u);
+			ctx->k->stream_collide_record(a, ctx->shape, ctx->stream, adj->moments + (t - ts) * 4u * N);
+		}
+		for(uint64_t t = te; t-- > ts;) {
+			fx::AdjointArgs g = adjoint_args(ctx, adj->lambda, rho, u, flags, t, fx_, fy_, fz_, F, alpha, adj->moments + (t - ts) * 4u * N,
+			                                 c.plane_x, t >= c.window_begin ? wobj : 0.0f, grad_alpha);
+			g.a.ext &= ~FX_UPDATE_FIELDS;
+			ctx->k->adjoint_step(g, ctx->shape, ctx->stream);
+		}
+		rc = check_launch("backward");
+		if(rc != FX_OK) return rc;
+	}
+	FX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+	FX_HIP(hipEventSynchronize(ctx->ev1));
+	if(ms) FX_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
 	return FX_OK;
 }
 

@@ -22,7 +22,7 @@ namespace fx {
 
 enum : uint32_t {
 	EXT_VOLUME_FORCE = 1u, EXT_FORCE_FIELD = 2u, EXT_EQUILIBRIUM_BOUNDARIES = 4u,
-	EXT_UPDATE_FIELDS = 8u, EXT_POROSITY = 16u
+	EXT_UPDATE_FIELDS = 8u, EXT_POROSITY = 16u, EXT_PRECISE_DENSITY = 32u
 };
 enum : uint32_t { ST_FP32 = 0u, ST_FP16S = 1u, ST_FP16C = 2u };
 constexpr uint32_t TYPE_S = 0x01u, TYPE_E = 0x02u, TYPE_BO = 0x03u, TYPE_SU = 0x38u, TYPE_G = 0x20u;
@@ -137,9 +137,9 @@ __device__ __forceinline__ void eq_pair(float* feq, int i, float rw, float rm1w,
 	feq[i] = __builtin_fmaf(rw, __builtin_fmaf(0.5f, a, uk), rm1w);
 	feq[i + 1] = __builtin_fmaf(rw, __builtin_fmaf(0.5f, a, -uk), rm1w);
 }
-// calculate_f_eq (src/kernel.cpp:1004-1061), DDF-shifted equilibrium
-template<int Q> __device__ __forceinline__ void calc_feq(float rho, float ux, float uy, float uz, float* feq) {
-	const float rhom1 = rho - 1.0f;
+// calculate_f_eq (src/kernel.cpp:1004-1061), DDF-shifted equilibrium; rhom1 = rho-1 (the
+// reference forms it as rho - 1.0f; with PRECISE_DENSITY it is the DDF sum itself)
+template<int Q> __device__ __forceinline__ void calc_feq(float rho, float rhom1, float ux, float uy, float uz, float* feq) {
 	const float c3 = -3.0f * ((ux * ux + uy * uy) + uz * uz);
 	uz *= 3.0f; ux *= 3.0f; uy *= 3.0f;
 	feq[0] = VS<Q>::w0 * __builtin_fmaf(rho, 0.5f * c3, rhom1);
@@ -163,13 +163,12 @@ template<int Q> __device__ __forceinline__ void calc_feq(float rho, float ux, fl
 }
 
 // calculate_rho_u (src/kernel.cpp:1063-1088): rho summed in slot order, +1 last (DDF
-// shifting); velocity sums in the reference's alternating order.
-template<int Q> __device__ __forceinline__ void calc_rho_u(const float* f, float& rho, float& ux, float& uy, float& uz) {
+// shifting); first moment j in the reference's alternating order; u = j/rho.
+template<int Q> __device__ __forceinline__ void calc_rho_j(const float* f, float& rho, float& jx, float& jy, float& jz) {
 	float r = f[0];
 #pragma unroll
 	for(int i = 1; i < Q; i++) r += f[i];
-	r += 1.0f;
-	float jx, jy, jz;
+	rho = r + 1.0f;
 	if constexpr(Q == 19) {
 		jx = f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[13] - f[14] + f[15] - f[16];
 		jy = f[3] - f[4] + f[7] - f[8] + f[11] - f[12] + f[14] - f[13] + f[17] - f[18];
@@ -179,10 +178,13 @@ template<int Q> __device__ __forceinline__ void calc_rho_u(const float* f, float
 		jy = f[3] - f[4] + f[7] - f[8] + f[11] - f[12] + f[14] - f[13] + f[17] - f[18] + f[19] - f[20] + f[21] - f[22] + f[24] - f[23] + f[25] - f[26];
 		jz = f[5] - f[6] + f[9] - f[10] + f[11] - f[12] + f[16] - f[15] + f[18] - f[17] + f[19] - f[20] + f[22] - f[21] + f[23] - f[24] + f[25] - f[26];
 	}
-	rho = r;
-	ux = jx / r;
-	uy = jy / r;
-	uz = jz / r;
+}
+template<int Q> __device__ __forceinline__ void calc_rho_u(const float* f, float& rho, float& ux, float& uy, float& uz) {
+	float jx, jy, jz;
+	calc_rho_j<Q>(f, rho, jx, jy, jz);
+	ux = jx / rho;
+	uy = jy / rho;
+	uz = jz / rho;
 }
 
 // c_i.v with the zero-coefficient terms left out, summed x, y, z in order
@@ -306,15 +308,15 @@ __global__ __launch_bounds__(BLOCK) void k_initialize(LbmArgs a) {
 		a.u[n] = 0.0f; a.u[a.N + n] = 0.0f; a.u[2ull * a.N + n] = 0.0f;
 	}
 	float feq[Q];
-	calc_feq<Q>(a.rho[n], ux, uy, uz, feq);
+	calc_feq<Q>(a.rho[n], a.rho[n] - 1.0f, ux, uy, uz, feq);
 	store_f<Q, ST>((typename Storage<ST>::E*)a.fi, a.N, s, 1u, feq);
 }
 
 // stream_collide (src/kernel.cpp:1454-1636).  EXT is a compile-time mask of the
 // extensions this instantiation may use; the runtime mask a.ext selects among them.
 // EXT = 0 is the plain scheme with every extension path compiled out.
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT>
-__global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD>
+__device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 	a.ext &= EXT;
 	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
 	if(x >= a.Nx) return;
@@ -329,10 +331,23 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
 	load_f<Q, ST>(fi, a.N, s, a.odd, fhn);
 
 	const bool isE = (a.ext & EXT_EQUILIBRIUM_BOUNDARIES) && bo == TYPE_E;
-	float rhon, uxn, uyn, uzn;
-	calc_rho_u<Q>(fhn, rhon, uxn, uyn, uzn);
+	float rhon, uxn, uyn, uzn, rhom1;
+	{
+		float jx, jy, jz;
+		calc_rho_j<Q>(fhn, rhon, jx, jy, jz);
+		if(a.ext & EXT_PRECISE_DENSITY) {
+			rhom1 = fhn[0];
+#pragma unroll
+			for(int i = 1; i < Q; i++) rhom1 += fhn[i];
+		} else rhom1 = rhon - 1.0f;
+		if constexpr(RECORD) { // pre-collision moments for the adjoint sweep
+			moments[n] = rhon; moments[a.N + n] = jx; moments[2ull * a.N + n] = jy; moments[3ull * a.N + n] = jz;
+		}
+		uxn = jx / rhon; uyn = jy / rhon; uzn = jz / rhon;
+	}
 	if(isE) { // preset density/velocity of equilibrium boundary cells
 		rhon = a.rho[n]; uxn = a.u[n]; uyn = a.u[a.N + n]; uzn = a.u[2ull * a.N + n];
+		rhom1 = rhon - 1.0f;
 	}
 	float fxn = a.fx, fyn = a.fy, fzn = a.fz;
 	if(a.ext & EXT_FORCE_FIELD) { fxn += a.F[n]; fyn += a.F[a.N + n]; fzn += a.F[2ull * a.N + n]; }
@@ -349,7 +364,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
 		a.rho[n] = rhon; a.u[n] = uxn; a.u[a.N + n] = uyn; a.u[2ull * a.N + n] = uzn;
 	}
 	float feq[Q];
-	calc_feq<Q>(rhon, uxn, uyn, uzn, feq);
+	calc_feq<Q>(rhon, rhom1, uxn, uyn, uzn, feq);
 	const float w = a.w;
 	if constexpr(!TRT) { // src/kernel.cpp:1595-1604
 		if(vf) {
@@ -391,6 +406,16 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
 		for(int i = 0; i < Q; i++) fhn[i] = out[i];
 	}
 	store_f<Q, ST>(fi, a.N, s, a.odd, fhn);
+}
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT>
+__global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
+	stream_collide_cell<Q, ST, TRT, EXT, false>(a, nullptr);
+}
+// stream_collide that also writes the pre-collision (rho, jx, jy, jz) of every fluid cell to
+// moments[c*N+n], c = 0..3 (16 B/cell): what the adjoint step needs of the forward state.
+template<int Q, uint32_t ST, bool TRT>
+__global__ __launch_bounds__(BLOCK) void k_stream_collide_record(LbmArgs a, float* moments) {
+	stream_collide_cell<Q, ST, TRT, EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY | EXT_PRECISE_DENSITY, true>(a, moments);
 }
 
 // update_fields (src/kernel.cpp:1794-1870)
@@ -457,6 +482,189 @@ __global__ __launch_bounds__(BLOCK) void k_mass_partial(LbmArgs a, double* parti
 		__syncthreads();
 	}
 	if(threadIdx.x == 0) partial[((uint64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// ------------------------------------------------------------------------------------
+// Adjoint (reverse) step.  The forward step maps the populations v loaded from the cell's
+// footprint locations L_k to v' = C(v) stored at S_m = L_opposite(m).  Its transpose reads
+// mu_m = lambda[S_m], forms nu = J_C^T mu (plus the objective weight on the outlet plane)
+// and writes lambda[L_k] = nu_k: the same footprint walk with loads and stores exchanged,
+// in place and race-free.  Solid cells are the identity.  J_C is the identity part of the
+// collision plus a correction through (rho, j) only, so it is evaluated from the
+// recorded forward moments with weighted moment sums of mu:
+//   SRT: v'_i = (1-w) v_i + w feq_i(rho,u) + (1-w/2) Fin_i(u,F)
+//   TRT: v'_i = v_i + (w+/2)(feq_i+feq_-i - v_i - v_-i) + (w-/2)(feq_i-feq_-i - v_i + v_-i) + Fin'_i
+// with u = clamp(k (j/rho + f/(2 rho))), k = 1/(1+alpha/2), F = f - alpha rho u (porosity).
+// The porosity derivative sum_m mu_m dv'_m/dalpha is accumulated into grad[n] (FP32).
+// ------------------------------------------------------------------------------------
+struct AdjointArgs {
+	LbmArgs a;               // a.fi = lambda (FP32, q*N), a.rho/a.u = TYPE_E presets
+	const float* moments;    // (rho, jx, jy, jz) of step t, 4*N
+	float* grad;             // d objective / d porosity, N
+	uint32_t plane_x;        // objective plane
+	float obj_weight;        // objective weight of step t (0 outside the averaging window)
+};
+
+template<int Q, bool TRT>
+__global__ __launch_bounds__(BLOCK) void k_adjoint_step(AdjointArgs g) {
+	const LbmArgs& a = g.a;
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+	if(x >= a.Nx) return;
+	const Site s = make_site(x, blockIdx.y, blockIdx.z, a.Nx, a.Ny, a.Nz);
+	const uint64_t n = s.n();
+	const uint32_t flagsn = a.flags[n];
+	const uint32_t bo = flagsn & TYPE_BO;
+	if(bo == TYPE_S || (flagsn & TYPE_SU) == TYPE_G) return;
+	float* lambda = (float*)a.fi;
+	float lam[Q], mu[Q];
+	load_f<Q, ST_FP32>(lambda, a.N, s, a.odd, lam); // lam[k] = lambda[L_k]
+	mu[0] = lam[0];
+#pragma unroll
+	for(int i = 1; i < Q; i += 2) { mu[i] = lam[i + 1]; mu[i + 1] = lam[i]; } // mu[m] = lambda[S_m]
+
+	const uint32_t ext = a.ext;
+	const bool isE = (ext & EXT_EQUILIBRIUM_BOUNDARIES) && bo == TYPE_E;
+	const bool vf = ext & (EXT_VOLUME_FORCE | EXT_POROSITY);
+	const bool por = ext & EXT_POROSITY;
+	float fxn = a.fx, fyn = a.fy, fzn = a.fz;
+	if(ext & EXT_FORCE_FIELD) { fxn += a.F[n]; fyn += a.F[a.N + n]; fzn += a.F[2ull * a.N + n]; }
+	const float alphan = por ? a.alpha[n] : 0.0f;
+	// forward quantities of step t, recomputed with the forward kernel's arithmetic
+	float rho, u0x, u0y, u0z;
+	if(isE) { rho = a.rho[n]; u0x = a.u[n]; u0y = a.u[a.N + n]; u0z = a.u[2ull * a.N + n]; }
+	else {
+		rho = g.moments[n];
+		u0x = g.moments[a.N + n] / rho; u0y = g.moments[2ull * a.N + n] / rho; u0z = g.moments[3ull * a.N + n] / rho;
+	}
+	float ubx = u0x, uby = u0y, ubz = u0z, k = 1.0f;
+	if(vf) {
+		const float rho2 = 0.5f / rho;
+		ubx = __builtin_fmaf(fxn, rho2, u0x); uby = __builtin_fmaf(fyn, rho2, u0y); ubz = __builtin_fmaf(fzn, rho2, u0z);
+		if(por) { k = 1.0f / (1.0f + 0.5f * alphan); ubx *= k; uby *= k; ubz *= k; }
+	}
+	const float ux = clampc(ubx), uy = clampc(uby), uz = clampc(ubz);
+	const float mx = (ubx > -DEF_C && ubx < DEF_C) ? 1.0f : 0.0f;
+	const float my = (uby > -DEF_C && uby < DEF_C) ? 1.0f : 0.0f;
+	const float mz = (ubz > -DEF_C && ubz < DEF_C) ? 1.0f : 0.0f;
+	float Fx = fxn, Fy = fyn, Fz = fzn;
+	if(por) { const float ar = -(alphan * rho); Fx = __builtin_fmaf(ar, ux, fxn); Fy = __builtin_fmaf(ar, uy, fyn); Fz = __builtin_fmaf(ar, uz, fzn); }
+
+	// weights of feq and Fin in sum_m mu_m v'_m
+	const float w = a.w;
+	float hp = 0.0f, hm = 0.0f, ctp = 0.0f, ctm = 0.0f;
+	if constexpr(TRT) {
+		const float wm = 1.0f / (0.1875f / (1.0f / w - 0.5f) + 0.5f);
+		hp = 0.5f * w; hm = 0.5f * wm;
+		ctp = __builtin_fmaf(w, -0.25f, 0.5f); ctm = __builtin_fmaf(wm, -0.25f, 0.5f);
+	}
+	const float ctau = __builtin_fmaf(w, -0.5f, 1.0f);
+	// weighted moments A (equilibrium weights) and B (forcing weights)
+	float A0 = 0.0f, Ax = 0.0f, Ay = 0.0f, Az = 0.0f, Axx = 0.0f, Ayy = 0.0f, Azz = 0.0f, Axy = 0.0f, Axz = 0.0f, Ayz = 0.0f;
+	float B0 = 0.0f, Bx = 0.0f, By = 0.0f, Bz = 0.0f, Bxx = 0.0f, Byy = 0.0f, Bzz = 0.0f, Bxy = 0.0f, Bxz = 0.0f, Byz = 0.0f;
+#pragma unroll
+	for(int i = 0; i < Q; i++) {
+		const int b = i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1);
+		float leq, lF;
+		if(isE) { leq = mu[i]; lF = 0.0f; }
+		else if constexpr(TRT) { leq = hp * (mu[i] + mu[b]) + hm * (mu[i] - mu[b]); lF = ctp * (mu[i] + mu[b]) + ctm * (mu[i] - mu[b]); }
+		else { leq = w * mu[i]; lF = ctau * mu[i]; }
+		const float we = weight<Q>(i) * leq, wf = weight<Q>(i) * lF;
+		const int cx = VS<Q>::cx[i], cy = VS<Q>::cy[i], cz = VS<Q>::cz[i];
+		A0 += we; B0 += wf;
+		if(cx) { Ax += cx * we; Bx += cx * wf; Axx += we; Bxx += wf; }
+		if(cy) { Ay += cy * we; By += cy * wf; Ayy += we; Byy += wf; }
+		if(cz) { Az += cz * we; Bz += cz * wf; Azz += we; Bzz += wf; }
+		if(cx && cy) { Axy += cx * cy * we; Bxy += cx * cy * wf; }
+		if(cx && cz) { Axz += cx * cz * we; Bxz += cx * cz * wf; }
+		if(cy && cz) { Ayz += cy * cz * we; Byz += cy * cz * wf; }
+	}
+	// d/du and d/drho of sum leq_i feq_i(rho,u) = rho (3 A1.u + 4.5 u.A2.u - 1.5 A0 u.u) + A0 (rho-1)
+	const float A2ux = Axx * ux + Axy * uy + Axz * uz, A2uy = Axy * ux + Ayy * uy + Ayz * uz, A2uz = Axz * ux + Ayz * uy + Azz * uz;
+	const float usq = ux * ux + uy * uy + uz * uz;
+	float gux = rho * (3.0f * Ax + 9.0f * A2ux - 3.0f * A0 * ux);
+	float guy = rho * (3.0f * Ay + 9.0f * A2uy - 3.0f * A0 * uy);
+	float guz = rho * (3.0f * Az + 9.0f * A2uz - 3.0f * A0 * uz);
+	float grho = 3.0f * (Ax * ux + Ay * uy + Az * uz) + 4.5f * (ux * A2ux + uy * A2uy + uz * A2uz) - 1.5f * A0 * usq + A0;
+	float galpha = 0.0f;
+	if(vf && !isE) { // sum lF_i Fin_i = 9 (F.B2.u + B1.F/3 - B0 u.F/3)
+		const float B2ux = Bxx * ux + Bxy * uy + Bxz * uz, B2uy = Bxy * ux + Byy * uy + Byz * uz, B2uz = Bxz * ux + Byz * uy + Bzz * uz;
+		const float B2Fx = Bxx * Fx + Bxy * Fy + Bxz * Fz, B2Fy = Bxy * Fx + Byy * Fy + Byz * Fz, B2Fz = Bxz * Fx + Byz * Fy + Bzz * Fz;
+		const float dFx = 9.0f * B2ux + 3.0f * Bx - 3.0f * B0 * ux; // d/dF at fixed u
+		const float dFy = 9.0f * B2uy + 3.0f * By - 3.0f * B0 * uy;
+		const float dFz = 9.0f * B2uz + 3.0f * Bz - 3.0f * B0 * uz;
+		gux += 9.0f * B2Fx - 3.0f * B0 * Fx; // d/du at fixed F
+		guy += 9.0f * B2Fy - 3.0f * B0 * Fy;
+		guz += 9.0f * B2Fz - 3.0f * B0 * Fz;
+		if(por) { // F = f - alpha rho u
+			const float ar = alphan * rho;
+			gux -= ar * dFx; guy -= ar * dFy; guz -= ar * dFz;
+			const float udF = ux * dFx + uy * dFy + uz * dFz;
+			grho -= alphan * udF;
+			galpha -= rho * udF;
+		}
+	}
+	// through u = clamp(ub), ub = k (j/rho + f/(2 rho)): dub/dj = k/rho, dub/drho = -ub/rho, dub/dalpha = -k ub/2
+	const float gbx = mx * gux, gby = my * guy, gbz = mz * guz;
+	const float gbub = gbx * ubx + gby * uby + gbz * ubz;
+	if(por) galpha -= 0.5f * k * gbub;
+	float nu[Q];
+	if(isE) {
+#pragma unroll
+		for(int i = 0; i < Q; i++) nu[i] = 0.0f; // v' = feq(presets): no dependence on v
+	} else {
+		const float kr = k / rho;
+		const float djx = gbx * kr, djy = gby * kr, djz = gbz * kr;
+		const float drho = grho - gbub / rho;
+		const float lin_a = TRT ? 1.0f - hp - hm : 1.0f - w, lin_b = TRT ? hm - hp : 0.0f;
+#pragma unroll
+		for(int i = 0; i < Q; i++) {
+			const int b = i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1);
+			float v = lin_a * mu[i] + drho;
+			if constexpr(TRT) v += lin_b * mu[b];
+			if(VS<Q>::cx[i]) v += VS<Q>::cx[i] * djx;
+			if(VS<Q>::cy[i]) v += VS<Q>::cy[i] * djy;
+			if(VS<Q>::cz[i]) v += VS<Q>::cz[i] * djz;
+			nu[i] = v;
+		}
+	}
+	if(x == g.plane_x && g.obj_weight != 0.0f) {
+#pragma unroll
+		for(int i = 1; i < Q; i++) if(VS<Q>::cx[i]) nu[i] += VS<Q>::cx[i] * g.obj_weight;
+	}
+	if(por) g.grad[n] += galpha;
+	float out[Q]; // lambda[L_k] = nu_k through store_f, which writes out[i] to S_i = L_opposite(i)
+	out[0] = nu[0];
+#pragma unroll
+	for(int i = 1; i < Q; i += 2) { out[i] = nu[i + 1]; out[i + 1] = nu[i]; }
+	store_f<Q, ST_FP32>(lambda, a.N, s, a.odd, out);
+}
+
+// Objective of the adjoint: x-momentum sum_i c_ix v_i of the populations loaded at step t,
+// per non-solid cell of the plane x = plane_x, summed in FP64 per block (one thread per
+// (y,z) cell of the plane; grid.x*256 >= Ny*Nz).
+template<int Q, uint32_t ST>
+__global__ __launch_bounds__(256) void k_objective_partial(LbmArgs a, uint32_t plane_x, double* partial) {
+	__shared__ double red[256];
+	const uint64_t m = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+	double v = 0.0;
+	if(m < (uint64_t)a.Ny * a.Nz) {
+		const uint32_t y = (uint32_t)(m % a.Ny), z = (uint32_t)(m / a.Ny);
+		const Site s = make_site(plane_x, y, z, a.Nx, a.Ny, a.Nz);
+		if((a.flags[s.n()] & TYPE_BO) != TYPE_S) {
+			float f[Q];
+			load_f<Q, ST>((const typename Storage<ST>::E*)a.fi, a.N, s, a.odd, f);
+			float r, jx, jy, jz;
+			calc_rho_j<Q>(f, r, jx, jy, jz);
+			v = (double)jx;
+		}
+	}
+	red[threadIdx.x] = v;
+	__syncthreads();
+	for(uint32_t k = 128u; k > 0u; k >>= 1) {
+		if(threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+		__syncthreads();
+	}
+	if(threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
 } // namespace fx

@@ -31,6 +31,9 @@ inline size_t occupancy_lds(uint32_t waves_per_simd, uint32_t block_threads, siz
 
 typedef void (*lattice_fn)(const LbmArgs&, const LaunchShape&, hipStream_t);
 typedef void (*mass_fn)(const LbmArgs&, const LaunchShape&, hipStream_t, double*);
+typedef void (*record_fn)(const LbmArgs&, const LaunchShape&, hipStream_t, float*);
+typedef void (*adjoint_fn)(const AdjointArgs&, const LaunchShape&, hipStream_t);
+typedef void (*objective_fn)(const LbmArgs&, uint32_t, hipStream_t, double*);
 
 struct LbmKernels {
 	lattice_fn initialize;
@@ -39,12 +42,15 @@ struct LbmKernels {
 	lattice_fn update_force_field;
 	mass_fn mass_partial;
 	uint32_t stream_collide_waves_cap;
+	record_fn stream_collide_record;
+	adjoint_fn adjoint_step;
+	objective_fn objective_partial; // grid of ceil(Ny*Nz/256) blocks, one partial each
 };
 
 const LbmKernels* kernels_q19(uint32_t storage, bool trt, bool plain);
 const LbmKernels* kernels_q27(uint32_t storage, bool trt, bool plain);
 
-constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY;
+constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY | EXT_PRECISE_DENSITY;
 
 // Kernel set of one (velocity set, storage, collision) variant; EXT = 0 instantiates the
 // extension-free kernels, EXT_ALL the ones that read the runtime extension mask.
@@ -54,12 +60,19 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static void update_fields(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_fields<Q, ST, EXT><<<s.grid, s.block, 0, st>>>(a); }
 	static void update_force_field(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_force_field<Q, ST><<<s.grid, s.block, 0, st>>>(a); }
 	static void mass_partial(const LbmArgs& a, const LaunchShape& s, hipStream_t st, double* p) { k_mass_partial<Q, ST><<<s.grid, s.block, 0, st>>>(a, p); }
+	static void stream_collide_record(const LbmArgs& a, const LaunchShape& s, hipStream_t st, float* m) { k_stream_collide_record<Q, ST, TRT><<<s.grid, s.block, 0, st>>>(a, m); }
+	static void adjoint_step(const AdjointArgs& g, const LaunchShape& s, hipStream_t st) { k_adjoint_step<Q, TRT><<<s.grid, s.block, 0, st>>>(g); }
+	static void objective_partial(const LbmArgs& a, uint32_t plane_x, hipStream_t st, double* p) {
+		const uint64_t cells = (uint64_t)a.Ny * a.Nz;
+		k_objective_partial<Q, ST><<<(unsigned)((cells + 255u) / 256u), 256, 0, st>>>(a, plane_x, p);
+	}
 	// Residency cap for stream_collide, in waves per SIMD (0 = none).  Measured on MI355X
 	// at 256^3: the plain FP32 kernel streams fastest at 3 waves/SIMD (fewer concurrent
 	// DDF streams); the 2-byte formats need full occupancy to keep enough bytes in flight.
 	static constexpr uint32_t waves_cap = (EXT == 0u && ST == ST_FP32) ? 3u : 0u;
 	static const LbmKernels* table() {
-		static const LbmKernels k = {initialize, stream_collide, update_fields, update_force_field, mass_partial, waves_cap};
+		static const LbmKernels k = {initialize, stream_collide, update_fields, update_force_field, mass_partial, waves_cap,
+		                             stream_collide_record, adjoint_step, objective_partial};
 		return &k;
 	}
 };

@@ -23,7 +23,8 @@
 //#define FORCE_FIELD // per-cell force field F and lbm.update_force_field()/object_force()
 //#define EQUILIBRIUM_BOUNDARIES // TYPE_E cells with preset density/velocity
 //#define UPDATE_FIELDS // rho/u written by every stream_collide
-//#define POROSITY // per-cell Brinkman porosity lbm.alpha (not in the reference)
+//#define POROSITY // per-cell Brinkman porosity lbm.porosity (not in the reference)
+//#define PRECISE_DENSITY // equilibrium takes rho-1 as the DDF sum (not in the reference; FP32 accuracy at tiny forces)
 
 #define TYPE_S 0x01 // (stationary) solid boundary
 #define TYPE_E 0x02 // equilibrium boundary (inflow/outflow)
@@ -40,6 +41,7 @@
 #undef FORCE_FIELD
 #undef EQUILIBRIUM_BOUNDARIES
 #undef POROSITY
+#undef PRECISE_DENSITY
 #endif
 
 #if defined(FP16S) && defined(FP16C)

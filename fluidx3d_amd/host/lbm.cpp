@@ -81,6 +81,9 @@ fx_lbm_config lbm_config(const uint Nx, const uint Ny, const uint Nz, const floa
 #ifdef POROSITY
 	c.extensions |= FX_POROSITY;
 #endif
+#ifdef PRECISE_DENSITY
+	c.extensions |= FX_PRECISE_DENSITY;
+#endif
 	c.w = fx_relaxation_rate(nu);
 	return c;
 }

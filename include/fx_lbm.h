@@ -53,6 +53,12 @@ extern "C" {
 #define FX_EQUILIBRIUM_BOUNDARIES 4u
 #define FX_UPDATE_FIELDS 8u
 #define FX_POROSITY 16u
+/* FX_PRECISE_DENSITY is new too: the equilibrium takes rho-1 as the sum of the shifted
+ * DDFs instead of forming rho = sum+1 and subtracting 1 again (src/kernel.cpp:1005,
+ * 1063-1066), which in FP32 quantises rho-1 to ~6e-8 per cell and step.  Off by default
+ * (the reference's arithmetic); the adjoint configurations use it because at body forces
+ * of 1e-6 per step that rounding is several percent of the flow's momentum input. */
+#define FX_PRECISE_DENSITY 32u
 
 /* cell type bits (src/defines.hpp:53-60) */
 #define FX_TYPE_S 0x01u
@@ -179,6 +185,48 @@ int fx_lbm_object_center_of_mass(fx_ctx* ctx, const uint8_t* flags, uint8_t mark
  * solid cell's slots.  t is accepted for symmetry and not used.  No reference
  * counterpart; the conservation check. */
 int fx_lbm_mass_sum(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t t, double* out);
+
+/* ---- differentiation (no reference counterpart; the reference has no adjoint) ----
+ *
+ * Objective J = mean over steps t in [window_begin, steps) of the x-momentum
+ * sum_i c_ix v_i of the populations loaded at step t, summed over the non-solid cells of
+ * the plane x = plane_x (the outlet flux of a channel).  fx_lbm_forward runs the steps
+ * from t = 0 (fi initialised by fx_lbm_initialize), stores a copy of fi every
+ * checkpoint_interval steps and returns J (FP64 sums).  fx_lbm_backward sweeps the
+ * checkpoint segments backward: each segment is recomputed forward recording the
+ * pre-collision moments (rho, j) of every step (16 B/cell/step), then the adjoint step
+ * (reverse Esoteric-Pull with the transposed collision, FP32) runs backward through it.
+ * grad_alpha[n] (N floats, device) receives dJ/dalpha[n] for FX_POROSITY. */
+typedef struct fx_adjoint_config {
+    uint64_t steps;               /* forward steps T, t = 0 .. T-1 */
+    uint64_t checkpoint_interval; /* DDF checkpoint every k steps; 0 selects ceil(sqrt(T)) */
+    uint64_t window_begin;        /* first step of the objective's averaging window (< steps) */
+    uint32_t plane_x;             /* objective plane, 0 <= plane_x < Nx */
+} fx_adjoint_config;
+
+typedef struct fx_adjoint fx_adjoint;
+
+/* device bytes fx_adjoint_create will allocate for this context and configuration */
+int fx_adjoint_memory(const fx_ctx* ctx, const fx_adjoint_config* cfg, uint64_t* bytes);
+int fx_adjoint_create(fx_ctx* ctx, const fx_adjoint_config* cfg, fx_adjoint** out);
+int fx_adjoint_destroy(fx_adjoint* adj);
+/* forward pass with checkpoints; *objective = J; *ms = GPU time (either may be NULL) */
+int fx_lbm_forward(fx_ctx* ctx, fx_adjoint* adj, void* fi, float* rho, float* u, const uint8_t* flags,
+                   float fx, float fy, float fz, const float* F, const float* alpha, double* objective, float* ms);
+/* checkpointed adjoint sweep after fx_lbm_forward; rho/u hold the TYPE_E presets */
+int fx_lbm_backward(fx_ctx* ctx, fx_adjoint* adj, const float* rho, const float* u, const uint8_t* flags,
+                    float fx, float fy, float fz, const float* F, const float* alpha, float* grad_alpha, float* ms);
+/* single-step primitives of the two passes (tests, custom schedules):
+ * stream_collide that also writes moments[c*N+n] = (rho, jx, jy, jz) of step t; */
+int fx_lbm_stream_collide_record(fx_ctx* ctx, void* fi, float* rho, float* u, const uint8_t* flags, uint64_t t,
+                                 float fx, float fy, float fz, const float* F, const float* alpha, float* moments);
+/* one adjoint step t on lambda (FP32, q*N, in place) with the moments of step t; adds
+ * obj_weight*c_ix on the plane x = plane_x and accumulates grad_alpha (FX_POROSITY) */
+int fx_lbm_adjoint_step(fx_ctx* ctx, float* lambda, const float* rho, const float* u, const uint8_t* flags, uint64_t t,
+                        float fx, float fy, float fz, const float* F, const float* alpha, const float* moments,
+                        uint32_t plane_x, float obj_weight, float* grad_alpha);
+/* plane x-momentum of the populations loaded at step t (FP64 sum), synchronous */
+int fx_lbm_objective_plane(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t t, uint32_t plane_x, double* out);
 
 /* ---- diagnostics ---- */
 

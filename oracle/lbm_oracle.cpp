@@ -35,7 +35,7 @@ namespace {
 
 enum : uint32_t {
 	EXT_VOLUME_FORCE = 1u, EXT_FORCE_FIELD = 2u, EXT_EQUILIBRIUM_BOUNDARIES = 4u,
-	EXT_UPDATE_FIELDS = 8u, EXT_POROSITY = 16u
+	EXT_UPDATE_FIELDS = 8u, EXT_POROSITY = 16u, EXT_PRECISE_DENSITY = 32u
 };
 enum : uint32_t { ST_FP32 = 0u, ST_FP16S = 1u, ST_FP16C = 2u, ST_FP64 = 3u };
 // cell type bits, src/defines.hpp:53-60 and src/lbm.cpp:399-411
@@ -207,6 +207,27 @@ template<> struct Lit<double> {
 	static double third() { return 1.0 / 3.0; }
 };
 
+// Dual number (value, derivative along one direction) for forward-mode derivatives of the
+// FP64 collision; used only by the adjoint oracle.
+struct Dual {
+	double v = 0.0, d = 0.0;
+	Dual() {}
+	Dual(double value, double deriv = 0.0) : v(value), d(deriv) {}
+};
+inline Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+inline Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+inline Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+inline Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+inline Dual operator/(Dual a, Dual b) { return Dual(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
+inline Dual& operator+=(Dual& a, Dual b) { a = a + b; return a; }
+inline Dual& operator*=(Dual& a, Dual b) { a = a * b; return a; }
+template<> struct Lit<Dual> {
+	static Dual c() { return Dual(0.57735027); }
+	static Dual third() { return Dual(1.0 / 3.0); }
+};
+inline Dual fmaT(Dual a, Dual b, Dual c) { return Dual(std::fma(a.v, b.v, c.v), a.d * b.v + a.v * b.d + c.d); }
+inline Dual clampT(Dual x, Dual lo, Dual hi) { return x.v < lo.v ? lo : x.v > hi.v ? hi : x; }
+
 inline float fmaT(float a, float b, float c) { return std::fmaf(a, b, c); }
 inline double fmaT(double a, double b, double c) { return std::fma(a, b, c); }
 template<typename T> inline T clampT(T x, T lo, T hi) { return std::fmin(std::fmax(x, lo), hi); } // OpenCL clamp
@@ -226,9 +247,10 @@ template<int Q, typename T> inline T project(int i, T x, T y, T z) {
 	return s;
 }
 
-// calculate_f_eq, src/kernel.cpp:1004-1061 (3D branch; DDF-shifted equilibrium)
-template<int Q, typename T> void calc_feq(const T rho, T ux, T uy, T uz, T* feq) {
-	const T rhom1 = rho - T(1);
+// calculate_f_eq, src/kernel.cpp:1004-1061 (3D branch; DDF-shifted equilibrium).  rhom1 is
+// rho-1: the reference forms it as rho - 1 (src/kernel.cpp:1005); with PRECISE_DENSITY the
+// caller passes the DDF sum itself, which keeps its relative precision.
+template<int Q, typename T> void calc_feq(const T rho, const T rhom1, T ux, T uy, T uz, T* feq) {
 	const T c3 = T(-3) * ((ux * ux + uy * uy) + uz * uz);
 	uz *= T(3); ux *= T(3); uy *= T(3);
 	feq[0] = VS<Q>::template w0<T>() * fmaT(rho, T(0.5) * c3, rhom1);
@@ -285,6 +307,88 @@ struct Cfg {
 	uint32_t Nx, Ny, Nz, q, trt, storage, ext, threads;
 	double w;
 };
+
+// The force block of stream_collide / update_fields, src/kernel.cpp:1549-1563 and
+// 1847-1859, plus the Brinkman porosity term (not in the reference):
+//   u = (j/rho + f*0.5/rho)/(1+alpha/2), F = f - alpha*rho*u  (implicit Guo-Brinkman).
+// With alpha = 0 the result is bit-identical to the reference's VOLUME_FORCE path.
+template<typename T> void force_block(uint32_t ext, T rhon, T& uxn, T& uyn, T& uzn, T& fxn, T& fyn, T& fzn, T a) {
+	const T cmax = Lit<T>::c();
+	if(ext & (EXT_VOLUME_FORCE | EXT_POROSITY)) {
+		const T rho2 = T(0.5) / rhon;
+		uxn = fmaT(fxn, rho2, uxn);
+		uyn = fmaT(fyn, rho2, uyn);
+		uzn = fmaT(fzn, rho2, uzn);
+		T ar = T(0);
+		if(ext & EXT_POROSITY) {
+			const T k = T(1) / (T(1) + T(0.5) * a);
+			uxn = uxn * k; uyn = uyn * k; uzn = uzn * k;
+			ar = a * rhon;
+		}
+		uxn = clampT(uxn, -cmax, cmax);
+		uyn = clampT(uyn, -cmax, cmax);
+		uzn = clampT(uzn, -cmax, cmax);
+		if(ext & EXT_POROSITY) {
+			fxn = fmaT(-ar, uxn, fxn);
+			fyn = fmaT(-ar, uyn, fyn);
+			fzn = fmaT(-ar, uzn, fzn);
+		}
+	} else {
+		uxn = clampT(uxn, -cmax, cmax);
+		uyn = clampT(uyn, -cmax, cmax);
+		uzn = clampT(uzn, -cmax, cmax);
+	}
+}
+
+// Collision of one cell, src/kernel.cpp:1474-1633: moments (or the TYPE_E presets), force
+// block, Guo forcing terms, equilibrium, SRT or TRT, in place on the loaded populations
+// fhn.  fxn.. is the external force of the cell (volume force plus force field).  If
+// fields is not null it receives rho, ux, uy, uz (the UPDATE_FIELDS values).
+template<int Q, typename T> void collide_cell(const Cfg& c, T* fhn, bool isE, T rhoE, T uxE, T uyE, T uzE,
+                                              T fxn, T fyn, T fzn, T alpha, T w, T* fields) {
+	const bool vf = c.ext & (EXT_VOLUME_FORCE | EXT_POROSITY);
+	T feq[Q], Fin[Q];
+	T rhon, uxn, uyn, uzn;
+	T rhom1;
+	if(isE) { rhon = rhoE; uxn = uxE; uyn = uyE; uzn = uzE; rhom1 = rhon - T(1); }
+	else {
+		calc_rho_u<Q, T>(fhn, &rhon, &uxn, &uyn, &uzn);
+		if(c.ext & EXT_PRECISE_DENSITY) { // rho-1 = sum of the shifted DDFs, same order as calc_rho_u
+			rhom1 = fhn[0];
+			for(int i = 1; i < Q; i++) rhom1 += fhn[i];
+		} else rhom1 = rhon - T(1);
+	}
+	force_block<T>(c.ext, rhon, uxn, uyn, uzn, fxn, fyn, fzn, alpha);
+	if(vf) calc_forcing<Q, T>(uxn, uyn, uzn, fxn, fyn, fzn, Fin);
+	else for(int i = 0; i < Q; i++) Fin[i] = T(0);
+	if(fields) { fields[0] = rhon; fields[1] = uxn; fields[2] = uyn; fields[3] = uzn; }
+	calc_feq<Q, T>(rhon, rhom1, uxn, uyn, uzn, feq);
+	if(!c.trt) { // SRT, src/kernel.cpp:1595-1604
+		if(vf) { const T c_tau = fmaT(w, T(-0.5), T(1)); for(int i = 0; i < Q; i++) Fin[i] *= c_tau; }
+		#pragma GCC unroll 32
+		for(int i = 0; i < Q; i++) fhn[i] = isE ? feq[i] : fmaT(T(1) - w, fhn[i], fmaT(w, feq[i], Fin[i]));
+	} else { // TRT, src/kernel.cpp:1605-1633
+		const T wp = w;
+		const T wm = T(1) / (T(0.1875) / (T(1) / w - T(0.5)) + T(0.5));
+		if(vf) {
+			const T c_taup = fmaT(wp, T(-0.25), T(0.5)), c_taum = fmaT(wm, T(-0.25), T(0.5));
+			T Fib[Q]; Fib[0] = Fin[0];
+			#pragma GCC unroll 32
+			for(int i = 1; i < Q; i += 2) { Fib[i] = Fin[i + 1]; Fib[i + 1] = Fin[i]; }
+			#pragma GCC unroll 32
+			for(int i = 0; i < Q; i++) Fin[i] = fmaT(c_taup, Fin[i] + Fib[i], c_taum * (Fin[i] - Fib[i]));
+		}
+		T fhb[Q], feb[Q];
+		fhb[0] = fhn[0]; feb[0] = feq[0];
+		#pragma GCC unroll 32
+		for(int i = 1; i < Q; i += 2) { fhb[i] = fhn[i + 1]; fhb[i + 1] = fhn[i]; feb[i] = feq[i + 1]; feb[i + 1] = feq[i]; }
+		#pragma GCC unroll 32
+		for(int i = 0; i < Q; i++) {
+			fhn[i] = isE ? feq[i] : fmaT(T(0.5) * wp, ((feq[i] - fhn[i]) + feb[i]) - fhb[i],
+			                             fmaT(T(0.5) * wm, ((feq[i] - feb[i]) - fhn[i]) + fhb[i], fhn[i] + Fin[i]));
+		}
+	}
+}
 
 template<typename T, int ST> struct Store;
 template<> struct Store<float, ST_FP32> {
@@ -363,94 +467,33 @@ template<typename T, int Q, int ST> struct Lattice {
 			for(uint64_t n = a; n < b; n++) {
 				neighbors(n, j);
 				if((flags[n] & TYPE_BO) == TYPE_S) { u[n] = T(0); u[N + n] = T(0); u[2 * N + n] = T(0); }
-				calc_feq<Q, T>(rho[n], u[n], u[N + n], u[2 * N + n], feq);
+				calc_feq<Q, T>(rho[n], rho[n] - T(1), u[n], u[N + n], u[2 * N + n], feq);
 				store_f(n, feq, fi, j, 1u);
 			}
 		});
 	}
 
-	// the force block of stream_collide / update_fields, src/kernel.cpp:1549-1563 and
-	// 1847-1859, plus the Brinkman porosity term (not in the reference):
-	//   u = (j/rho + f*0.5/rho)/(1+alpha/2), F = f - alpha*rho*u  (implicit Guo-Brinkman).
-	// With alpha = 0 the result is bit-identical to the reference's VOLUME_FORCE path.
-	void force_block(uint64_t n, T rhon, T& uxn, T& uyn, T& uzn, T& fxn, T& fyn, T& fzn, const T* alpha) const {
-		const T cmax = Lit<T>::c();
-		if(c.ext & (EXT_VOLUME_FORCE | EXT_POROSITY)) {
-			const T rho2 = T(0.5) / rhon;
-			uxn = fmaT(fxn, rho2, uxn);
-			uyn = fmaT(fyn, rho2, uyn);
-			uzn = fmaT(fzn, rho2, uzn);
-			T ar = T(0);
-			if(c.ext & EXT_POROSITY) {
-				const T a = alpha[n];
-				const T k = T(1) / (T(1) + T(0.5) * a);
-				uxn = uxn * k; uyn = uyn * k; uzn = uzn * k;
-				ar = a * rhon;
-			}
-			uxn = clampT(uxn, -cmax, cmax);
-			uyn = clampT(uyn, -cmax, cmax);
-			uzn = clampT(uzn, -cmax, cmax);
-			if(c.ext & EXT_POROSITY) {
-				fxn = fmaT(-ar, uxn, fxn);
-				fyn = fmaT(-ar, uyn, fyn);
-				fzn = fmaT(-ar, uzn, fzn);
-			}
-		} else {
-			uxn = clampT(uxn, -cmax, cmax);
-			uyn = clampT(uyn, -cmax, cmax);
-			uzn = clampT(uzn, -cmax, cmax);
-		}
-	}
+	T relaxation_rate() const { return (T)(std::is_same<T, float>::value ? (double)(float)c.w : c.w); }
 
 	// stream_collide kernel, src/kernel.cpp:1454-1636
 	void stream_collide(E* fi, T* rho, T* u, const uint8_t* flags, uint64_t t, T fx, T fy, T fz, const T* F, const T* alpha) const {
 		const bool eq = c.ext & EXT_EQUILIBRIUM_BOUNDARIES;
-		const bool vf = c.ext & (EXT_VOLUME_FORCE | EXT_POROSITY);
-		const T w = (T)(std::is_same<T, float>::value ? (double)(float)c.w : c.w);
+		const T w = relaxation_rate();
 		parallel_cells(N, c.threads, [&](uint64_t a, uint64_t b) {
-			uint64_t j[Q]; T fhn[Q], feq[Q], Fin[Q];
+			uint64_t j[Q]; T fhn[Q], fields[4];
 			for(uint64_t n = a; n < b; n++) {
 				const uint8_t flagsn = flags[n];
 				const uint8_t bo = flagsn & TYPE_BO, su = flagsn & TYPE_SU;
 				if(bo == TYPE_S || su == TYPE_G) continue;
 				neighbors(n, j);
 				load_f(n, fhn, fi, j, t);
-				T rhon, uxn, uyn, uzn;
 				const bool isE = eq && bo == TYPE_E;
-				if(isE) { rhon = rho[n]; uxn = u[n]; uyn = u[N + n]; uzn = u[2 * N + n]; }
-				else calc_rho_u<Q, T>(fhn, &rhon, &uxn, &uyn, &uzn);
 				T fxn = fx, fyn = fy, fzn = fz;
 				if(c.ext & EXT_FORCE_FIELD) { fxn += F[n]; fyn += F[N + n]; fzn += F[2 * N + n]; }
-				force_block(n, rhon, uxn, uyn, uzn, fxn, fyn, fzn, alpha);
-				if(vf) calc_forcing<Q, T>(uxn, uyn, uzn, fxn, fyn, fzn, Fin);
-				else for(int i = 0; i < Q; i++) Fin[i] = T(0);
-				if((c.ext & EXT_UPDATE_FIELDS) && !isE) { rho[n] = rhon; u[n] = uxn; u[N + n] = uyn; u[2 * N + n] = uzn; }
-				calc_feq<Q, T>(rhon, uxn, uyn, uzn, feq);
-				if(!c.trt) { // SRT, src/kernel.cpp:1595-1604
-					if(vf) { const T c_tau = fmaT(w, T(-0.5), T(1)); for(int i = 0; i < Q; i++) Fin[i] *= c_tau; }
-					#pragma GCC unroll 32
-					for(int i = 0; i < Q; i++) fhn[i] = isE ? feq[i] : fmaT(T(1) - w, fhn[i], fmaT(w, feq[i], Fin[i]));
-				} else { // TRT, src/kernel.cpp:1605-1633
-					const T wp = w;
-					const T wm = T(1) / (T(0.1875) / (T(1) / w - T(0.5)) + T(0.5));
-					if(vf) {
-						const T c_taup = fmaT(wp, T(-0.25), T(0.5)), c_taum = fmaT(wm, T(-0.25), T(0.5));
-						T Fib[Q]; Fib[0] = Fin[0];
-						#pragma GCC unroll 32
-						for(int i = 1; i < Q; i += 2) { Fib[i] = Fin[i + 1]; Fib[i + 1] = Fin[i]; }
-						#pragma GCC unroll 32
-						for(int i = 0; i < Q; i++) Fin[i] = fmaT(c_taup, Fin[i] + Fib[i], c_taum * (Fin[i] - Fib[i]));
-					}
-					T fhb[Q], feb[Q];
-					fhb[0] = fhn[0]; feb[0] = feq[0];
-					#pragma GCC unroll 32
-					for(int i = 1; i < Q; i += 2) { fhb[i] = fhn[i + 1]; fhb[i + 1] = fhn[i]; feb[i] = feq[i + 1]; feb[i + 1] = feq[i]; }
-					#pragma GCC unroll 32
-					for(int i = 0; i < Q; i++) {
-						fhn[i] = isE ? feq[i] : fmaT(T(0.5) * wp, ((feq[i] - fhn[i]) + feb[i]) - fhb[i],
-						                             fmaT(T(0.5) * wm, ((feq[i] - feb[i]) - fhn[i]) + fhb[i], fhn[i] + Fin[i]));
-					}
-				}
+				const T an = (c.ext & EXT_POROSITY) ? alpha[n] : T(0);
+				collide_cell<Q, T>(c, fhn, isE, isE ? rho[n] : T(0), isE ? u[n] : T(0), isE ? u[N + n] : T(0), isE ? u[2 * N + n] : T(0),
+				                   fxn, fyn, fzn, an, w, fields);
+				if((c.ext & EXT_UPDATE_FIELDS) && !isE) { rho[n] = fields[0]; u[n] = fields[1]; u[N + n] = fields[2]; u[2 * N + n] = fields[3]; }
 				store_f(n, fhn, fi, j, t);
 			}
 		});
@@ -471,7 +514,7 @@ template<typename T, int Q, int ST> struct Lattice {
 				calc_rho_u<Q, T>(fhn, &rhon, &uxn, &uyn, &uzn);
 				T fxn = fx, fyn = fy, fzn = fz;
 				if(c.ext & EXT_FORCE_FIELD) { fxn += F[n]; fyn += F[N + n]; fzn += F[2 * N + n]; }
-				force_block(n, rhon, uxn, uyn, uzn, fxn, fyn, fzn, alpha);
+				force_block<T>(c.ext, rhon, uxn, uyn, uzn, fxn, fyn, fzn, (c.ext & EXT_POROSITY) ? alpha[n] : T(0));
 				if(!(eq && bo == TYPE_E)) { rho[n] = rhon; u[n] = uxn; u[N + n] = uyn; u[2 * N + n] = uzn; }
 			}
 		});
@@ -489,6 +532,74 @@ template<typename T, int Q, int ST> struct Lattice {
 				calc_rho_u<Q, T>(fhn, &Fb, &fxv, &fyv, &fzv);
 				const T s = T(2) * Fb;
 				F[n] = s * fxv; F[N + n] = s * fyv; F[2 * N + n] = s * fzv;
+			}
+		});
+	}
+
+	// Objective of the adjoint: x-momentum of the populations loaded at step t, summed over
+	// the non-solid cells of the plane x = plane_x (FP64).
+	double objective_plane(const E* fi, const uint8_t* flags, uint64_t t, uint32_t plane_x) const {
+		double s = 0.0;
+		uint64_t j[Q]; T fhn[Q];
+		for(uint32_t z = 0; z < c.Nz; z++) for(uint32_t y = 0; y < c.Ny; y++) {
+			const uint64_t n = (uint64_t)plane_x + ((uint64_t)y + (uint64_t)z * c.Ny) * c.Nx;
+			if((flags[n] & TYPE_BO) == TYPE_S) continue;
+			neighbors(n, j);
+			load_f(n, fhn, fi, j, t);
+			for(int i = 1; i < Q; i++) s += (double)VS<Q>::cx[i] * (double)fhn[i];
+		}
+		return s;
+	}
+
+	// Adjoint of stream_collide at step t (FP64 oracle).  The forward step maps the
+	// populations v loaded at the cell's footprint locations L_k to v' = C(v), stored at
+	// S_m = L_{opposite(m)}.  Its transpose reads mu_m = lambda[S_m], forms
+	// nu_k = sum_m mu_m dv'_m/dv_k (+ the objective's c_kx weight on the plane) and writes
+	// lambda[L_k] = nu_k; solid cells are the identity.  The derivatives are taken by
+	// forward-mode (dual numbers) through collide_cell, the same code as the forward step,
+	// one direction per population and one for the porosity; grad[n] += sum_m mu_m dv'_m/dalpha.
+	void adjoint_step(const E* f_state, double* lambda, const uint8_t* flags, uint64_t t, double fx, double fy, double fz,
+	                  const double* F, const double* alpha, const double* rho, const double* u, uint32_t plane_x, double obj_weight, double* grad) const {
+		const bool eq = c.ext & EXT_EQUILIBRIUM_BOUNDARIES;
+		const Dual w = Dual(c.w);
+		parallel_cells(N, c.threads, [&](uint64_t a, uint64_t b) {
+			uint64_t j[Q]; double v[Q], lam[Q], mu[Q], nu[Q], out[Q];
+			Dual vd[Q];
+			for(uint64_t n = a; n < b; n++) {
+				const uint8_t flagsn = flags[n];
+				const uint8_t bo = flagsn & TYPE_BO, su = flagsn & TYPE_SU;
+				if(bo == TYPE_S || su == TYPE_G) continue;
+				neighbors(n, j);
+				load_f(n, v, f_state, j, t);
+				load_f(n, lam, lambda, j, t); // lam[k] = lambda[L_k]
+				mu[0] = lam[0];
+				for(int i = 1; i < Q; i += 2) { mu[i] = lam[i + 1]; mu[i + 1] = lam[i]; } // mu[m] = lambda[S_m]
+				const bool isE = eq && bo == TYPE_E;
+				double fxn = fx, fyn = fy, fzn = fz;
+				if(c.ext & EXT_FORCE_FIELD) { fxn += F[n]; fyn += F[N + n]; fzn += F[2 * N + n]; }
+				const double an = (c.ext & EXT_POROSITY) ? alpha[n] : 0.0;
+				const Dual rE = isE ? Dual(rho[n]) : Dual(0.0), uxE = isE ? Dual(u[n]) : Dual(0.0), uyE = isE ? Dual(u[N + n]) : Dual(0.0), uzE = isE ? Dual(u[2 * N + n]) : Dual(0.0);
+				for(int k = 0; k < Q; k++) {
+					for(int i = 0; i < Q; i++) vd[i] = Dual(v[i], i == k ? 1.0 : 0.0);
+					collide_cell<Q, Dual>(c, vd, isE, rE, uxE, uyE, uzE, Dual(fxn), Dual(fyn), Dual(fzn), Dual(an), w, nullptr);
+					double s = 0.0;
+					for(int m = 0; m < Q; m++) s += mu[m] * vd[m].d;
+					nu[k] = s;
+				}
+				if(c.ext & EXT_POROSITY) {
+					for(int i = 0; i < Q; i++) vd[i] = Dual(v[i], 0.0);
+					collide_cell<Q, Dual>(c, vd, isE, rE, uxE, uyE, uzE, Dual(fxn), Dual(fyn), Dual(fzn), Dual(an, 1.0), w, nullptr);
+					double s = 0.0;
+					for(int m = 0; m < Q; m++) s += mu[m] * vd[m].d;
+					grad[n] += s;
+				}
+				if(obj_weight != 0.0 && (uint32_t)(n % c.Nx) == plane_x) {
+					for(int k = 1; k < Q; k++) nu[k] += obj_weight * (double)VS<Q>::cx[k];
+				}
+				// lambda[L_k] = nu[k]: store_f writes out[i] to S_i = L_opposite(i)
+				out[0] = nu[0];
+				for(int i = 1; i < Q; i += 2) { out[i] = nu[i + 1]; out[i + 1] = nu[i]; }
+				store_f(n, out, lambda, j, t);
 			}
 		});
 	}
@@ -660,6 +771,25 @@ int fxo_update_force_field(const fxo_cfg* c, const void* fi, const uint8_t* flag
 	if(!valid(c)) return -1;
 	return dispatch<DoForceField>(*c, fi, flags, t, F);
 }
+// FP64 only: adjoint of stream_collide at step t (see Lattice::adjoint_step).
+int fxo_adjoint_step(const fxo_cfg* c, const void* f_state, void* lambda, const uint8_t* flags, uint64_t t, double fx, double fy, double fz,
+                     const void* F, const void* alpha, uint32_t plane_x, double obj_weight, void* grad, const void* rho, const void* u) {
+	if(!valid(c) || c->storage != ST_FP64) return -1;
+	if((c->ext & EXT_POROSITY) && (!alpha || !grad)) return -2;
+	if((c->ext & EXT_FORCE_FIELD) && !F) return -2;
+	if((c->ext & EXT_EQUILIBRIUM_BOUNDARIES) && (!rho || !u)) return -2;
+	if(c->q == 19u) Lattice<double, 19, ST_FP64>(*c).adjoint_step((const double*)f_state, (double*)lambda, flags, t, fx, fy, fz, (const double*)F, (const double*)alpha, (const double*)rho, (const double*)u, plane_x, obj_weight, (double*)grad);
+	else Lattice<double, 27, ST_FP64>(*c).adjoint_step((const double*)f_state, (double*)lambda, flags, t, fx, fy, fz, (const double*)F, (const double*)alpha, (const double*)rho, (const double*)u, plane_x, obj_weight, (double*)grad);
+	return 0;
+}
+// FP64 only: objective plane sum at step t
+int fxo_objective_plane(const fxo_cfg* c, const void* fi, const uint8_t* flags, uint64_t t, uint32_t plane_x, double* out) {
+	if(!valid(c) || c->storage != ST_FP64 || plane_x >= c->Nx) return -1;
+	*out = c->q == 19u ? Lattice<double, 19, ST_FP64>(*c).objective_plane((const double*)fi, flags, t, plane_x)
+	                   : Lattice<double, 27, ST_FP64>(*c).objective_plane((const double*)fi, flags, t, plane_x);
+	return 0;
+}
+
 // pre-collision populations of every cell at step t (float, or double for FP64)
 int fxo_load_populations(const fxo_cfg* c, const void* fi, uint64_t t, void* out) {
 	if(!valid(c)) return -1;

@@ -23,6 +23,7 @@ EXT_FORCE_FIELD = 2
 EXT_EQUILIBRIUM_BOUNDARIES = 4
 EXT_UPDATE_FIELDS = 8
 EXT_POROSITY = 16
+EXT_PRECISE_DENSITY = 32
 
 TYPE_S = 0x01
 TYPE_E = 0x02
@@ -68,10 +69,9 @@ def lib():
         L.fxo_object_force.argtypes = [cp, vp, u8p, ctypes.c_uint8, dp]
         L.fxo_object_torque.argtypes = [cp, vp, u8p, ctypes.c_uint8, ctypes.c_float, ctypes.c_float, ctypes.c_float, dp]
         L.fxo_object_center_of_mass.argtypes = [cp, u8p, ctypes.c_uint8, dp]
-        if hasattr(L, "fxo_adjoint_step"):
-            L.fxo_adjoint_step.argtypes = [cp, vp, vp, u8p, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
-                                           ctypes.c_double, vp, vp, ctypes.c_uint32, ctypes.c_double, vp]
-            L.fxo_objective_plane.argtypes = [cp, vp, u8p, ctypes.c_uint64, ctypes.c_uint32, dp]
+        L.fxo_adjoint_step.argtypes = [cp, vp, vp, u8p, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, vp, vp, ctypes.c_uint32, ctypes.c_double, vp, vp, vp]
+        L.fxo_objective_plane.argtypes = [cp, vp, u8p, ctypes.c_uint64, ctypes.c_uint32, dp]
         _LIB = L
     return _LIB
 
@@ -193,6 +193,42 @@ class OracleLBM:
         out = np.empty(self.q * self.N, dtype=self.real)
         assert lib().fxo_load_populations(ctypes.byref(self.cfg), _ptr(self.fi), self.t, _ptr(out)) == 0
         return out.reshape(self.q, self.N)
+
+    # -- adjoint (FP64 storage only) --------------------------------------------------
+    def objective_plane(self, plane_x):
+        """x-momentum of the populations loaded at the current step, summed over the
+        non-solid cells of the plane x = plane_x."""
+        out = ctypes.c_double()
+        assert lib().fxo_objective_plane(ctypes.byref(self.cfg), _ptr(self.fi), self.flags_ptr, self.t, plane_x, ctypes.byref(out)) == 0
+        return out.value
+
+    def run_objective(self, steps, window_begin, plane_x, keep_states=False):
+        """Step `steps` times from t=0; J = mean over steps t >= window_begin of the plane
+        momentum loaded at step t.  Optionally keeps the DDF array before every step."""
+        states = []
+        acc = 0.0
+        for t in range(steps):
+            if keep_states:
+                states.append(self.fi.copy())
+            if t >= window_begin:
+                acc += self.objective_plane(plane_x)
+            self.step(1)
+        return acc / (steps - window_begin), states
+
+    def adjoint_gradient(self, steps, window_begin, plane_x):
+        """dJ/dporosity by the FP64 adjoint over the stored forward states (small grids).
+        Call after initialize(); returns (J, grad)."""
+        assert self.storage == "FP64"
+        J, states = self.run_objective(steps, window_begin, plane_x, keep_states=True)
+        lam = np.zeros_like(self.fi)
+        grad = np.zeros(self.N, dtype=np.float64)
+        wobj = 1.0 / (steps - window_begin)
+        for t in range(steps - 1, -1, -1):
+            rc = lib().fxo_adjoint_step(ctypes.byref(self.cfg), _ptr(states[t]), _ptr(lam), self.flags_ptr, t,
+                                        self.fx, self.fy, self.fz, _ptr(self.F), _ptr(self.alpha), plane_x,
+                                        wobj if t >= window_begin else 0.0, _ptr(grad), _ptr(self.rho), _ptr(self.u))
+            assert rc == 0, rc
+        return J, grad
 
     def ddf_values(self):
         """DDF array decoded to float (shifted populations, slot layout as stored)."""

@@ -58,3 +58,45 @@ def sphere_flags(Nx, Ny, Nz, center, radius, marker=TYPE_S):
 def kinetic_energy(rho, u, N):
     ux, uy, uz = u[:N].astype(np.float64), u[N:2 * N].astype(np.float64), u[2 * N:].astype(np.float64)
     return 0.5 * float(np.sum(rho.astype(np.float64) * (ux * ux + uy * uy + uz * uz)))
+
+
+def porosity_noise(Nx, Ny, Nz, x0, x1, sigma=4.0, seed=42):
+    """Seeded Gaussian-smoothed noise in [0,1] on the x-slab [x0, x1), 0 elsewhere: normal
+    noise (numpy PCG64, seed) smoothed with a periodic Gaussian of width sigma cells (FFT,
+    float64), then min-max normalised over the slab."""
+    rng = np.random.default_rng(seed)
+    noise = rng.standard_normal((Nz, Ny, Nx))
+    k = [np.fft.fftfreq(m) for m in (Nz, Ny, Nx)]
+    kz, ky, kx = np.meshgrid(*k, indexing="ij")
+    gauss = np.exp(-2.0 * np.pi ** 2 * sigma ** 2 * (kx ** 2 + ky ** 2 + kz ** 2))
+    smooth = np.real(np.fft.ifftn(np.fft.fftn(noise) * gauss))
+    out = np.zeros((Nz, Ny, Nx))
+    slab = smooth[:, :, x0:x1]
+    out[:, :, x0:x1] = (slab - slab.min()) / (slab.max() - slab.min())
+    return out.ravel()
+
+
+def channel_c4(Nx=128, Ny=64, Nz=64, sigma=4.0, seed=42):
+    """Adjoint channel: TYPE_S walls at y=0 and y=Ny-1, periodic in x and z, fluid at rest,
+    Brinkman porosity in [0,1] from porosity_noise on the middle third of x (zero on the
+    walls).  Returns (rho, u, flags, porosity) as float32/uint8 arrays."""
+    N = Nx * Ny * Nz
+    _, y, _ = coords(Nx, Ny, Nz)
+    flags = np.zeros(N, dtype=np.uint8)
+    flags[(y == 0) | (y == Ny - 1)] = TYPE_S
+    alpha = porosity_noise(Nx, Ny, Nz, Nx // 3, (2 * Nx) // 3, sigma, seed)
+    alpha[flags == TYPE_S] = 0.0
+    return (np.ones(N, dtype=np.float32), np.zeros(3 * N, dtype=np.float32), flags, alpha.astype(np.float32))
+
+
+# adjoint configuration of the 128x64x64 channel: nu=1/6, body force 1e-6 in x, 2000 steps,
+# objective = outlet-plane (x = Nx-1) x-momentum averaged over the last 500 steps
+C4 = {"Nx": 128, "Ny": 64, "Nz": 64, "nu": 1.0 / 6.0, "fx": 1e-6, "steps": 2000, "window_begin": 1500,
+      "plane_x": 127, "checkpoint_interval": 100, "fd_cells": 32, "fd_seed": 7, "fd_eps": 1e-3}
+
+
+def fd_cells(flags, alpha, count, seed):
+    """Seeded choice of porous fluid cells for the finite-difference check."""
+    candidates = np.flatnonzero((alpha > 0.0) & ((flags & TYPE_S) == 0))
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(candidates, count, replace=False))
