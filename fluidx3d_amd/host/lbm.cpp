@@ -146,6 +146,7 @@ void LBM::construct(const uint nx, const uint ny, const uint nz, const uint dx, 
 #endif
 #ifdef POROSITY
 	porosity.allocate(this, N, 1u, "porosity", 0.0f);
+	porosity_gradient.allocate(this, N, 1u, "porosity_gradient", 0.0f);
 #endif
 }
 
@@ -157,6 +158,8 @@ LBM::~LBM() {
 #endif
 #ifdef POROSITY
 	porosity.release();
+	porosity_gradient.release();
+	if(adjoint) fx_adjoint_destroy(adjoint);
 #endif
 	if(ctx) fx_ctx_destroy(ctx); // frees fi with the context
 	ctx = nullptr;
@@ -276,6 +279,36 @@ float3 LBM::object_torque(const float3& rotation_center, const uchar flag_marker
 	double s[3];
 	fx_check(fx_lbm_object_torque(ctx, (const float*)F.device_buffer(), (const uint8_t*)flags.device_buffer(), flag_marker, rotation_center.x, rotation_center.y, rotation_center.z, s), "object_torque");
 	return float3((float)s[0], (float)s[1], (float)s[2]);
+}
+#endif
+
+#ifdef POROSITY
+double LBM::forward(const ulong steps, const ulong window_begin, const uint plane_x, const ulong checkpoint_interval) {
+	if(adjoint) { fx_adjoint_destroy(adjoint); adjoint = nullptr; }
+	fx_adjoint_config c{steps, checkpoint_interval, window_begin, plane_x};
+	fx_check(fx_adjoint_create(ctx, &c, &adjoint), "adjoint workspace");
+	initialize();
+	double J = 0.0;
+	fx_check(fx_lbm_forward(ctx, adjoint, fi, (float*)rho.device_buffer(), (float*)u.device_buffer(), (const uint8_t*)flags.device_buffer(), fx, fy, fz,
+#ifdef FORCE_FIELD
+	                        (const float*)F.device_buffer(),
+#else
+	                        nullptr,
+#endif
+	                        (const float*)porosity.device_buffer(), &J, &forward_ms), "forward");
+	t = steps;
+	return J;
+}
+void LBM::backward() {
+	if(!adjoint) print_error("LBM::backward() needs a preceding LBM::forward()");
+	fx_check(fx_lbm_backward(ctx, adjoint, (const float*)rho.device_buffer(), (const float*)u.device_buffer(), (const uint8_t*)flags.device_buffer(), fx, fy, fz,
+#ifdef FORCE_FIELD
+	                         (const float*)F.device_buffer(),
+#else
+	                         nullptr,
+#endif
+	                         (const float*)porosity.device_buffer(), (float*)porosity_gradient.device_buffer(), &backward_ms), "backward");
+	fx_check(fx_buf_read(ctx, porosity_gradient.device_buffer(), 0ull, porosity_gradient.capacity(), porosity_gradient.data(), 1), "reading the gradient");
 }
 #endif
 

@@ -76,6 +76,7 @@ class LBM {
 	fx_ctx* ctx = nullptr;
 	void* fi = nullptr; // DDFs, device only
 	fx_lbm_config cfg{};
+	fx_adjoint* adjoint = nullptr;
 
 	void construct(const uint Nx, const uint Ny, const uint Nz, const uint Dx, const uint Dy, const uint Dz, const float nu, const float fx, const float fy, const float fz,
 	               const float sigma, const float alpha, const float beta, const uint particles_N, const float particles_rho);
@@ -92,6 +93,7 @@ public:
 #endif
 #ifdef POROSITY
 	Memory_Container<float> porosity; // Brinkman porosity alpha per cell (not in the reference)
+	Memory_Container<float> porosity_gradient; // dJ/dporosity after backward()
 #endif
 
 	LBM(const uint Nx, const uint Ny, const uint Nz, const uint Dx, const uint Dy, const uint Dz, const float nu, const float fx = 0.0f, const float fy = 0.0f, const float fz = 0.0f, const float sigma = 0.0f, const float alpha = 0.0f, const float beta = 0.0f, const uint particles_N = 0u, const float particles_rho = 0.0f);
@@ -116,6 +118,16 @@ public:
 	float3 object_torque(const float3& rotation_center, const uchar flag_marker = TYPE_S);
 #endif
 	double total_mass(); // FP64 mass invariant of the DDFs (not in the reference)
+#ifdef POROSITY
+	// Differentiation (not in the reference).  forward() re-initializes from the host fields,
+	// runs `steps` steps from t=0 with DDF checkpoints every checkpoint_interval steps
+	// (0 = ceil(sqrt(steps))) and returns J = mean over t >= window_begin of the x-momentum
+	// through the plane x = plane_x.  backward() runs the checkpointed adjoint and leaves
+	// dJ/dporosity in porosity_gradient (host and device).
+	double forward(const ulong steps, const ulong window_begin, const uint plane_x, const ulong checkpoint_interval = 0ull);
+	void backward();
+	float backward_ms = 0.0f, forward_ms = 0.0f; // GPU time of the last forward()/backward()
+#endif
 
 	uint get_Nx() const { return Nx; }
 	uint get_Ny() const { return Ny; }

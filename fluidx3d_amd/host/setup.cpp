@@ -91,8 +91,38 @@ void main_setup() {
 #endif
 }
 
+#elif defined(SCENE_POROUS_CHANNEL)
+#ifndef CHANNEL_SCALE
+#define CHANNEL_SCALE 1u
+#endif
+// Porous channel of the adjoint: walls at y=0 and y=Ny-1, periodic x/z, body force 1e-6,
+// porosity (Brinkman) in the middle third of x; J = outlet x-momentum over the last quarter
+// of the run; prints J and dJ/dporosity at a few cells.  Needs VOLUME_FORCE and POROSITY.
+void main_setup() {
+	const uint s = CHANNEL_SCALE;
+	LBM lbm(128u * s, 64u * s, 64u * s, 1.0f / 6.0f, 1E-6f, 0.0f, 0.0f);
+	const uint Nx = lbm.get_Nx(), Ny = lbm.get_Ny();
+	uint seed = 42u;
+	for(ulong n = 0ull; n < lbm.get_N(); n++) {
+		uint x = 0u, y = 0u, z = 0u; lbm.coordinates(n, x, y, z);
+		if(y == 0u || y == Ny - 1u) { lbm.flags[n] = TYPE_S; continue; }
+		lbm.porosity[n] = (x >= Nx / 3u && x < 2u * Nx / 3u) ? random(seed, 1.0f) : 0.0f;
+	}
+	const ulong steps = 2000ull * s;
+	const double J = lbm.forward(steps, steps - steps / 4ull, Nx - 1u, 0ull);
+	lbm.backward();
+	double norm = 0.0;
+	for(ulong n = 0ull; n < lbm.get_N(); n++) norm += sq((double)lbm.porosity_gradient[n]);
+	char line[256];
+	std::snprintf(line, sizeof(line), "J=%.12e |dJ/dporosity|=%.6e forward %.1f ms backward %.1f ms", J, std::sqrt(norm), (double)lbm.forward_ms, (double)lbm.backward_ms);
+	println(line);
+#ifdef FX_VTK
+	lbm.porosity_gradient.write_host_to_vtk("porosity_gradient.vtk", false);
+#endif
+}
+
 #else
 void main_setup() {
-	print_error("no scene selected: compile with -DBENCHMARK or -DSCENE_TAYLOR_GREEN or -DSCENE_SPHERE");
+	print_error("no scene selected: compile with -DBENCHMARK, -DSCENE_TAYLOR_GREEN, -DSCENE_SPHERE or -DSCENE_POROUS_CHANNEL");
 }
 #endif

@@ -92,3 +92,53 @@ def test_cpp_sphere_scene(tmp_path):
     Fx, Fy, Fz, Cd = (float(v) for v in m.groups())
     print("\n" + r.stdout.strip().splitlines()[-1])
     assert np.isfinite([Fx, Fy, Fz, Cd]).all() and Fx > 0 and abs(Fy) < 0.1 * Fx and abs(Fz) < 0.1 * Fx
+
+
+def lcg_channel_porosity(Nx, Ny, Nz, seed=42):
+    """The porous channel scene's porosity: the C99 LCG of utilities.hpp (the reference's
+    random(), src/utilities.hpp:244-250) drawn in cell order over the fluid cells of the
+    middle third of x; walls at y=0 and y=Ny-1."""
+    N = Nx * Ny * Nz
+    n = np.arange(N)
+    x, y = n % Nx, (n // Nx) % Ny
+    wall = (y == 0) | (y == Ny - 1)
+    porous = ~wall & (x >= Nx // 3) & (x < 2 * Nx // 3)
+    draws = np.empty(int(porous.sum()), np.float32)
+    s = seed
+    for i in range(draws.size):
+        s = (1103515245 * s + 12345) % 2147483648
+        draws[i] = np.float32(s)
+    alpha = np.zeros(N, np.float32)
+    alpha[porous] = np.float32(1.0) * draws * np.float32(4.65661287e-10)
+    flags = np.where(wall, 1, 0).astype(np.uint8)
+    return flags, alpha
+
+
+@pytest.mark.gpu
+def test_cpp_porous_channel_backward_matches_python_path(tmp_path):
+    """LBM::forward/LBM::backward from C++ (128x64x64 porous channel, 2000 steps, objective
+    over the last 500, sqrt(T) checkpoints) == fluidx3d_amd.adjoint.Adjoint on the same
+    inputs: same objective to the printed digits and a bit-identical gradient field."""
+    import fluidx3d_amd as fx
+    from fluidx3d_amd.adjoint import Adjoint
+    r = subprocess.run([binary("porous_channel")], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"J=([-0-9.e+]+) \|dJ/dporosity\|=([-0-9.e+]+) forward ([0-9.]+) ms backward ([0-9.]+) ms", r.stdout)
+    assert m, r.stdout
+    J_cpp = float(m.group(1))
+    print("\n" + m.group(0))
+    dims, g_cpp = read_vtk(os.path.join(tmp_path, "porosity_gradient.vtk"))
+    Nx, Ny, Nz = 128, 64, 64
+    assert dims == [Nx, Ny, Nz]
+    flags, alpha = lcg_channel_porosity(Nx, Ny, Nz)
+    lbm = fx.LBM(Nx, Ny, Nz, 1.0 / 6.0, 1e-6, 0.0, 0.0, extensions=("VOLUME_FORCE", "POROSITY", "PRECISE_DENSITY"))
+    lbm.flags.host[:] = flags
+    lbm.porosity.host[:] = alpha
+    adj = Adjoint(lbm, 2000, 1500, Nx - 1, 0)
+    J = adj.forward()
+    g = adj.backward()
+    adj.close()
+    lbm.close()
+    assert abs(J - J_cpp) <= 1e-11 * abs(J)
+    assert np.array_equal(g_cpp[:, 0], g)
+    assert np.linalg.norm(g) > 0
