@@ -39,7 +39,8 @@ struct fx_ctx {
 	hipStream_t stream = nullptr;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
 	const fx::LbmKernels* k = nullptr;
-	fx::LaunchShape shape{};
+	fx::LaunchShape shape{};    // one lane per cell: every kernel but stream_collide
+	fx::LaunchShape shape_sc{}; // stream_collide (cells per lane, residency cap)
 	double* partial = nullptr; // reduction partials (device)
 	uint64_t partial_cap = 0;  // in doubles
 	double* result = nullptr;  // 4 doubles (device)
@@ -322,16 +323,15 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 	ctx->device = device;
 	ctx->cfg = *cfg;
 	ctx->N = N;
-	const bool plain = cfg->extensions == 0u;
-	ctx->k = cfg->velocity_set == FX_D3Q19 ? fx::kernels_q19(cfg->storage, cfg->collision == FX_TRT, plain)
-	                                        : fx::kernels_q27(cfg->storage, cfg->collision == FX_TRT, plain);
+	ctx->k = fx::select_kernels(cfg->velocity_set, cfg->storage, cfg->collision == FX_TRT, cfg->extensions);
 	ctx->shape = fx::lattice_shape(cfg->Nx, cfg->Ny, cfg->Nz);
+	ctx->shape_sc = fx::lattice_shape(cfg->Nx, cfg->Ny, cfg->Nz, ctx->k->stream_collide_cells);
 	hipError_t e = hipSetDevice(device);
 	if(e == hipSuccess) {
 		hipDeviceProp_t prop;
 		e = hipGetDeviceProperties(&prop, device);
 		if(e == hipSuccess && ctx->k->stream_collide_waves_cap)
-			ctx->shape.lds = fx::occupancy_lds(ctx->k->stream_collide_waves_cap, ctx->shape.block.x, prop.maxSharedMemoryPerMultiProcessor, prop.sharedMemPerBlock);
+			ctx->shape_sc.lds = fx::occupancy_lds(ctx->k->stream_collide_waves_cap, ctx->shape_sc.block.x, prop.maxSharedMemoryPerMultiProcessor, prop.sharedMemPerBlock);
 	}
 	if(e == hipSuccess) {
 		size_t free_b = 0, total_b = 0;
@@ -444,7 +444,7 @@ int fx_lbm_stream_collide(fx_ctx* ctx, void* fi, float* rho, float* u, const uin
 	if(rc != FX_OK) return rc;
 	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
 	const fx::LbmArgs a = make_args(ctx, fi, rho, u, flags, t, fx_, fy_, fz_, F, alpha);
-	ctx->k->stream_collide(a, ctx->shape, ctx->stream);
+	ctx->k->stream_collide(a, ctx->shape_sc, ctx->stream);
 	return check_launch("stream_collide");
 }
 
@@ -457,7 +457,7 @@ int fx_lbm_run(fx_ctx* ctx, void* fi, float* rho, float* u, const uint8_t* flags
 	fx::LbmArgs a = make_args(ctx, fi, rho, u, flags, t0, fx_, fy_, fz_, F, alpha);
 	for(uint64_t s = 0; s < steps; s++) {
 		a.odd = (uint32_t)((t0 + s) & 1ull);
-		ctx->k->stream_collide(a, ctx->shape, ctx->stream);
+		ctx->k->stream_collide(a, ctx->shape_sc, ctx->stream);
 	}
 	const int lc = check_launch("stream_collide");
 	if(lc != FX_OK) return lc;
@@ -674,7 +674,7 @@ int fx_lbm_forward(fx_ctx* ctx, fx_adjoint* adj, void* fi, float* rho, float* u,
 			if(rc != FX_OK) return rc;
 		}
 		a.odd = (uint32_t)(t & 1u);
-		ctx->k->stream_collide(a, ctx->shape, ctx->stream);
+		ctx->k->stream_collide(a, ctx->shape_sc, ctx->stream);
 	}
 	rc = check_launch("forward");
 	if(rc != FX_OK) return rc;

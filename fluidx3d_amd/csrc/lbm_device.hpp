@@ -65,7 +65,9 @@ template<uint32_t ST> struct Storage;
 // for the 2-byte formats, which keep regular loads).
 template<> struct Storage<ST_FP32> {
 	typedef float E;
-	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
+	__device__ static __forceinline__ float raw(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
+	__device__ static __forceinline__ float decode(float x) { return x; }
+	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return raw(p, o); }
 	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { __builtin_nontemporal_store(x, elem(p, o)); }
 };
 // FP16S (src/lbm.cpp:410-414): IEEE binary16 of x*2^15, round to nearest even
@@ -74,6 +76,7 @@ template<> struct Storage<ST_FP16S> {
 	typedef uint16_t E;
 	__device__ static __forceinline__ float decode(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h) * 3.0517578E-5f; }
 	__device__ static __forceinline__ uint16_t encode(float x) { return __builtin_bit_cast(uint16_t, (_Float16)(x * 32768.0f)); }
+	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return *elem(p, o); }
 	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(*elem(p, o)); }
 	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { *elem(p, o) = encode(x); }
 };
@@ -101,6 +104,7 @@ template<> struct Storage<ST_FP16C> {
 		const uint32_t mag = e > 112u ? normal : (e > 100u ? denormal : 0u);
 		return (uint16_t)(((b >> 16) & 0x8000u) | mag);
 	}
+	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return *elem(p, o); }
 	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(*elem(p, o)); }
 	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { *elem(p, o) = encode(x); }
 };
@@ -264,17 +268,24 @@ __device__ __forceinline__ Site make_site(uint32_t x, uint32_t y, uint32_t z, ui
 // the parity of t.  Loads and stores touch the same locations, so the update is in place
 // and race-free.
 template<int Q, uint32_t ST>
-__device__ __forceinline__ void load_f(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, float* f) {
+__device__ __forceinline__ void load_raw(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, typename Storage<ST>::E* r) {
 	typedef Storage<ST> S;
 	const uint64_t n0 = s.R[4];
-	f[0] = S::load(fi + n0, s.x);
+	r[0] = S::raw(fi + n0, s.x);
 #pragma unroll
 	for(int i = 1; i < Q; i += 2) {
 		const uint64_t sa = odd ? (uint64_t)i : (uint64_t)(i + 1);
 		const uint64_t sb = odd ? (uint64_t)(i + 1) : (uint64_t)i;
-		f[i] = S::load(fi + sa * N + n0, s.x);
-		f[i + 1] = S::load(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]), s.X[VS<Q>::cx[i] + 1]);
+		r[i] = S::raw(fi + sa * N + n0, s.x);
+		r[i + 1] = S::raw(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]), s.X[VS<Q>::cx[i] + 1]);
 	}
+}
+template<int Q, uint32_t ST>
+__device__ __forceinline__ void load_f(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, float* f) {
+	typename Storage<ST>::E r[Q];
+	load_raw<Q, ST>(fi, N, s, odd, r);
+#pragma unroll
+	for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
 }
 template<int Q, uint32_t ST>
 __device__ __forceinline__ void store_f(typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, const float* f) {
@@ -315,21 +326,13 @@ __global__ __launch_bounds__(BLOCK) void k_initialize(LbmArgs a) {
 // stream_collide (src/kernel.cpp:1454-1636).  EXT is a compile-time mask of the
 // extensions this instantiation may use; the runtime mask a.ext selects among them.
 // EXT = 0 is the plain scheme with every extension path compiled out.
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD>
-__device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
-	a.ext &= EXT;
-	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-	if(x >= a.Nx) return;
-	const Site s = make_site(x, blockIdx.y, blockIdx.z, a.Nx, a.Ny, a.Nz);
-	const uint64_t n = s.n();
-	const uint32_t flagsn = a.flags[n];
-	const uint32_t bo = flagsn & TYPE_BO;
-	if(bo == TYPE_S || (flagsn & TYPE_SU) == TYPE_G) return; // solid cells are mailboxes
-	typedef typename Storage<ST>::E E;
-	E* fi = (E*)a.fi;
-	float fhn[Q];
-	load_f<Q, ST>(fi, a.N, s, a.odd, fhn);
-
+//
+// collide_cell: everything between load_f and store_f for the fluid or TYPE_E cell n;
+// fhn holds the streamed-in DDFs on entry and the post-collision DDFs on return.
+// Writes to rho/u (UPDATE_FIELDS) and moments (RECORD) happen only for live cells; the
+// arithmetic of a dead cell is discarded by the caller.
+template<int Q, bool TRT, uint32_t EXT, bool RECORD>
+__device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint32_t bo, bool live, float* fhn, float* moments) {
 	const bool isE = (a.ext & EXT_EQUILIBRIUM_BOUNDARIES) && bo == TYPE_E;
 	float rhon, uxn, uyn, uzn, rhom1;
 	{
@@ -340,7 +343,7 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 #pragma unroll
 			for(int i = 1; i < Q; i++) rhom1 += fhn[i];
 		} else rhom1 = rhon - 1.0f;
-		if constexpr(RECORD) { // pre-collision moments for the adjoint sweep
+		if(RECORD && live) { // pre-collision moments for the adjoint sweep
 			moments[n] = rhon; moments[a.N + n] = jx; moments[2ull * a.N + n] = jy; moments[3ull * a.N + n] = jz;
 		}
 		uxn = jx / rhon; uyn = jy / rhon; uzn = jz / rhon;
@@ -360,7 +363,7 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 #pragma unroll
 		for(int i = 0; i < Q; i++) Fin[i] = 0.0f;
 	}
-	if((a.ext & EXT_UPDATE_FIELDS) && !isE) {
+	if((a.ext & EXT_UPDATE_FIELDS) && !isE && live) {
 		a.rho[n] = rhon; a.u[n] = uxn; a.u[a.N + n] = uyn; a.u[2ull * a.N + n] = uzn;
 	}
 	float feq[Q];
@@ -405,17 +408,69 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 #pragma unroll
 		for(int i = 0; i < Q; i++) fhn[i] = out[i];
 	}
-	store_f<Q, ST>(fi, a.N, s, a.odd, fhn);
 }
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT>
+
+// C cells per lane: wave v of the block covers the C*64 consecutive cells starting at
+// blockIdx.x*C*blockDim.x + v*C*64, lane l takes cells l, l+64, ..., l+(C-1)*64 of that
+// range.  The loads of all C cells are straight-line code issued before any store (their
+// footprints are disjoint, so the reordering is exact): C*Q loads of one wave in flight,
+// with the row-contiguous pattern of C waves of one cell each.  A lane whose C cells are
+// all solid or outside the row leaves at once; otherwise it also loads (and discards) the
+// DDFs of its dead cells, at clamped in-bounds addresses, so that no branch separates the
+// loads.  Only live cells store.
+template<uint32_t C> __device__ __forceinline__ uint32_t cell_x(uint32_t c) {
+	return blockIdx.x * (blockDim.x * C) + (threadIdx.x & ~63u) * C + c * 64u + (threadIdx.x & 63u);
+}
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD, uint32_t C>
+__device__ __forceinline__ void stream_collide_cells(LbmArgs a, float* moments) {
+	a.ext &= EXT;
+	typedef typename Storage<ST>::E E;
+	E* fi = (E*)a.fi;
+	Site s[C];
+	uint32_t bo[C];
+	bool live[C];
+	uint32_t flagsn[C];
+#pragma unroll
+	for(uint32_t c = 0u; c < C; c++) {
+		const uint32_t x = cell_x<C>(c);
+		live[c] = x < a.Nx;
+		s[c] = make_site(live[c] ? x : a.Nx - 1u, blockIdx.y, blockIdx.z, a.Nx, a.Ny, a.Nz);
+	}
+#pragma unroll
+	for(uint32_t c = 0u; c < C; c++) flagsn[c] = a.flags[s[c].n()];
+	bool any = false;
+#pragma unroll
+	for(uint32_t c = 0u; c < C; c++) {
+		bo[c] = flagsn[c] & TYPE_BO;
+		live[c] = live[c] && !(bo[c] == TYPE_S || (flagsn[c] & TYPE_SU) == TYPE_G); // solid cells are mailboxes
+		any = any || live[c];
+	}
+	if(!any) return;
+	typename Storage<ST>::E r[C][Q];
+#pragma unroll
+	for(uint32_t c = 0u; c < C; c++) load_raw<Q, ST>(fi, a.N, s[c], a.odd, r[c]);
+	// Keep every load above this point: left alone, the scheduler may decode each 16-bit
+	// value right after its load (to save registers) and wait for it, one load at a time.
+	__builtin_amdgcn_sched_barrier(0);
+	float f[C][Q];
+#pragma unroll
+	for(uint32_t c = 0u; c < C; c++)
+#pragma unroll
+		for(int i = 0; i < Q; i++) f[c][i] = Storage<ST>::decode(r[c][i]);
+#pragma unroll
+	for(uint32_t c = 0u; c < C; c++) collide_cell<Q, TRT, EXT, RECORD>(a, s[c].n(), bo[c], live[c], f[c], moments);
+#pragma unroll
+	for(uint32_t c = 0u; c < C; c++) if(live[c]) store_f<Q, ST>(fi, a.N, s[c], a.odd, f[c]);
+}
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, uint32_t C = 1u>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
-	stream_collide_cell<Q, ST, TRT, EXT, false>(a, nullptr);
+	stream_collide_cells<Q, ST, TRT, EXT, false, C>(a, nullptr);
 }
 // stream_collide that also writes the pre-collision (rho, jx, jy, jz) of every fluid cell to
 // moments[c*N+n], c = 0..3 (16 B/cell): what the adjoint step needs of the forward state.
 template<int Q, uint32_t ST, bool TRT>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide_record(LbmArgs a, float* moments) {
-	stream_collide_cell<Q, ST, TRT, EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY | EXT_PRECISE_DENSITY, true>(a, moments);
+	stream_collide_cells<Q, ST, TRT, EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY | EXT_PRECISE_DENSITY, true, 1u>(a, moments);
 }
 
 // update_fields (src/kernel.cpp:1794-1870)

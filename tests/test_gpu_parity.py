@@ -20,6 +20,10 @@ EXT_SETS = {
     "forces": ("VOLUME_FORCE", "FORCE_FIELD", "EQUILIBRIUM_BOUNDARIES", "UPDATE_FIELDS"),
     "porosity": ("POROSITY",),
     "precise": ("VOLUME_FORCE", "POROSITY", "PRECISE_DENSITY", "EQUILIBRIUM_BOUNDARIES"),
+    # subsets with a specialised stream_collide of their own
+    "equilibrium": ("EQUILIBRIUM_BOUNDARIES",),
+    "volume_force": ("VOLUME_FORCE",),
+    "drag": ("EQUILIBRIUM_BOUNDARIES", "FORCE_FIELD"),
 }
 EXT_BITS = {"VOLUME_FORCE": 1, "FORCE_FIELD": 2, "EQUILIBRIUM_BOUNDARIES": 4, "UPDATE_FIELDS": 8, "POROSITY": 16,
             "PRECISE_DENSITY": 32}
@@ -66,11 +70,16 @@ def compare_state(lbm, ref, oracle, storage, tag):
     assert bad.size == 0, f"{tag}: {bad.size} DDF values differ, first slot/cell {divmod(int(bad[0]), lbm.N)}: gpu {g[bad[0]]!r} oracle {o[bad[0]]!r}"
 
 
+GRIDS = {"20x12x10": (20, 12, 10), "150x6x5": (150, 6, 5)}
+
+
+@pytest.mark.parametrize("grid", GRIDS)
 @pytest.mark.parametrize("q,collision,storage,ext", list(itertools.product((19, 27), ("SRT", "TRT"), ("FP32", "FP16S", "FP16C"), EXT_SETS)))
-def test_stream_collide_matches_oracle(oracle, q, collision, storage, ext):
+def test_stream_collide_matches_oracle(oracle, q, collision, storage, ext, grid):
     """Random fields with scattered solids (and TYPE_E planes, forces, porosity): 7 steps,
-    both parities, ragged x-rows (Nx=20 in a 64-lane block)."""
-    Nx, Ny, Nz = 20, 12, 10
+    both parities, ragged x-rows: Nx=20 in one 64-lane block, and Nx=150, which with two
+    cells per lane leaves a block with a full wave, a partly live wave and dead second cells."""
+    Nx, Ny, Nz = GRIDS[grid]
     names = EXT_SETS[ext]
     rho, u, flags = scenes.random_scene(Nx, Ny, Nz, seed=q + len(names), equilibrium_faces="EQUILIBRIUM_BOUNDARIES" in names)
     N = Nx * Ny * Nz
