@@ -19,6 +19,7 @@ FX_EQUILIBRIUM_BOUNDARIES = 4
 FX_UPDATE_FIELDS = 8
 FX_POROSITY = 16
 FX_PRECISE_DENSITY = 32
+FX_FIELD_FI, FX_FIELD_RHO_U_FLAGS, FX_FIELD_FLAGS, FX_FIELD_F = 0, 1, 2, 3
 
 STORAGE = {"FP32": FX_FP32, "FP16S": FX_FP16S, "FP16C": FX_FP16C}
 COLLISION = {"SRT": FX_SRT, "TRT": FX_TRT}
@@ -35,7 +36,7 @@ EXPORTS = [
     "fx_lbm_reset_force_field", "fx_lbm_object_force", "fx_lbm_object_torque", "fx_lbm_object_center_of_mass",
     "fx_lbm_mass_sum", "fx_codec_roundtrip", "fx_bench_copy", "fx_adjoint_memory", "fx_adjoint_create",
     "fx_adjoint_destroy", "fx_lbm_forward", "fx_lbm_backward", "fx_lbm_stream_collide_record", "fx_lbm_adjoint_step",
-    "fx_lbm_objective_plane",
+    "fx_lbm_objective_plane", "fx_transfer_bytes", "fx_lbm_transfer_extract", "fx_lbm_transfer_insert",
 ]
 
 
@@ -116,6 +117,10 @@ def load():
     L.fx_lbm_adjoint_step.argtypes = [vp, vp, vp, vp, vp, u64, f32, f32, f32, vp, vp, vp, u32, f32, vp]
     L.fx_lbm_objective_plane.argtypes = [vp, vp, vp, u64, u32, dp]
     L.fx_bench_copy.argtypes = [vp, vp, vp, u64, u32, ctypes.POINTER(f32)]
+    L.fx_transfer_bytes.restype = u64
+    L.fx_transfer_bytes.argtypes = [ctypes.POINTER(Config), u32, u32]
+    L.fx_lbm_transfer_extract.argtypes = [vp, u32, u32, u64, vp, vp, vp, vp, vp, vp, vp]
+    L.fx_lbm_transfer_insert.argtypes = [vp, u32, u32, u64, vp, vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
 

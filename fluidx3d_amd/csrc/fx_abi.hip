@@ -40,7 +40,7 @@ struct fx_ctx {
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
 	const fx::LbmKernels* k = nullptr;
 	fx::LaunchShape shape{};    // one lane per cell: every kernel but stream_collide
-	fx::LaunchShape shape_sc{}; // stream_collide (cells per lane, residency cap)
+	fx::LaunchShape shape_sc{}; // stream_collide (residency cap, XCD-mapped grid)
 	double* partial = nullptr; // reduction partials (device)
 	uint64_t partial_cap = 0;  // in doubles
 	double* result = nullptr;  // 4 doubles (device)
@@ -90,6 +90,7 @@ fx::LbmArgs make_args(const fx_ctx* ctx, const void* fi, const float* rho, const
 	a.Nx = ctx->cfg.Nx; a.Ny = ctx->cfg.Ny; a.Nz = ctx->cfg.Nz;
 	a.ext = ctx->cfg.extensions;
 	a.odd = (uint32_t)(t & 1ull);
+	a.xchunks = 0u; // set by the stream_collide launcher from its shape
 	a.w = ctx->cfg.w;
 	a.fx = fx_; a.fy = fy_; a.fz = fz_;
 	return a;
@@ -249,6 +250,38 @@ __global__ __launch_bounds__(256) void k_copy16(const float4* __restrict__ src, 
 // ---------------------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------------------
+// Field halo transfer (rho/u/flags, flags, F): interior layers 1 and N-2 -> halo layers 0
+// and N-1 of the neighbours.  Face cell a as in k_transfer_fi; layout per field: rho/u/flags
+// = 4 float planes of A then A flag bytes; flags = A bytes; F = 3 float planes of A.
+template<int AXIS, bool INSERT>
+__global__ void k_transfer_field(uint32_t field, char* buffer_p, char* buffer_m, float* rho, float* u, uint8_t* flags, float* F,
+                                 uint32_t Nx, uint32_t Ny, uint32_t Nz) {
+	const uint32_t Nax = AXIS == 0 ? Nx : AXIS == 1 ? Ny : Nz;
+	const uint64_t A = AXIS == 0 ? (uint64_t)Ny * Nz : AXIS == 1 ? (uint64_t)Nx * Nz : (uint64_t)Nx * Ny;
+	const uint64_t a = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if(a >= A) return;
+	const uint64_t N = (uint64_t)Nx * Ny * Nz;
+	for(int sd = 0; sd < 2; sd++) {
+		char* buffer = sd == 0 ? buffer_p : buffer_m;
+		const uint32_t layer = INSERT ? (sd == 0 ? Nax - 1u : 0u) : (sd == 0 ? Nax - 2u : 1u);
+		uint32_t x, y, z;
+		fx::face_cell<AXIS>((uint32_t)a, layer, Nx, Ny, x, y, z);
+		const uint64_t n = ((uint64_t)z * Ny + y) * Nx + x;
+		float* bf = (float*)buffer;
+		if(field == FX_FIELD_RHO_U_FLAGS) {
+			uint8_t* bb = (uint8_t*)buffer + 16u * A;
+			if(INSERT) { rho[n] = bf[a]; u[n] = bf[A + a]; u[N + n] = bf[2u * A + a]; u[2u * N + n] = bf[3u * A + a]; flags[n] = bb[a]; }
+			else { bf[a] = rho[n]; bf[A + a] = u[n]; bf[2u * A + a] = u[N + n]; bf[3u * A + a] = u[2u * N + n]; bb[a] = flags[n]; }
+		} else if(field == FX_FIELD_FLAGS) {
+			uint8_t* bb = (uint8_t*)buffer;
+			if(INSERT) flags[n] = bb[a]; else bb[a] = flags[n];
+		} else { // FX_FIELD_F
+			if(INSERT) { F[n] = bf[a]; F[N + n] = bf[A + a]; F[2u * N + n] = bf[2u * A + a]; }
+			else { bf[a] = F[n]; bf[A + a] = F[N + n]; bf[2u * A + a] = F[2u * N + n]; }
+		}
+	}
+}
+
 extern "C" {
 
 const char* fx_last_error(void) { return g_last_error.c_str(); }
@@ -325,7 +358,7 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 	ctx->N = N;
 	ctx->k = fx::select_kernels(cfg->velocity_set, cfg->storage, cfg->collision == FX_TRT, cfg->extensions);
 	ctx->shape = fx::lattice_shape(cfg->Nx, cfg->Ny, cfg->Nz);
-	ctx->shape_sc = fx::lattice_shape(cfg->Nx, cfg->Ny, cfg->Nz, ctx->k->stream_collide_cells);
+	ctx->shape_sc = fx::stream_shape(cfg->Nx, cfg->Ny, cfg->Nz, ctx->k->stream_collide_xcd_map);
 	hipError_t e = hipSetDevice(device);
 	if(e == hipSuccess) {
 		hipDeviceProp_t prop;
@@ -727,6 +760,62 @@ int fx_lbm_backward(fx_ctx* ctx, fx_adjoint* adj, const float* rho, const float*
 	FX_HIP(hipEventSynchronize(ctx->ev1));
 	if(ms) FX_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
 	return FX_OK;
+}
+
+static uint64_t face_area(const fx_lbm_config* cfg, uint32_t direction) {
+	return direction == 0u ? (uint64_t)cfg->Ny * cfg->Nz : direction == 1u ? (uint64_t)cfg->Nx * cfg->Nz : (uint64_t)cfg->Nx * cfg->Ny;
+}
+
+uint64_t fx_transfer_bytes(const fx_lbm_config* cfg, uint32_t field, uint32_t direction) {
+	if(!cfg || direction > 2u || !valid_enums(cfg)) return 0u;
+	const uint64_t A = face_area(cfg, direction);
+	switch(field) {
+		case FX_FIELD_FI: return A * (cfg->velocity_set == FX_D3Q19 ? 5u : 9u) * (cfg->storage == FX_FP32 ? 4u : 2u);
+		case FX_FIELD_RHO_U_FLAGS: return A * 17u;
+		case FX_FIELD_FLAGS: return A;
+		case FX_FIELD_F: return A * 12u;
+		default: return 0u;
+	}
+}
+
+static int transfer(fx_ctx* ctx, uint32_t field, uint32_t direction, uint64_t t, void* bp, void* bm, void* fi, float* rho, float* u,
+                    uint8_t* flags, float* F, bool insert) {
+	if(!ctx || !bp || !bm) return fail(FX_ERR_ARG, "null argument");
+	if(direction > 2u) return fail(FX_ERR_ARG, "direction must be 0 (x), 1 (y) or 2 (z)");
+	const fx_lbm_config& c = ctx->cfg;
+	const uint32_t Nax = direction == 0u ? c.Nx : direction == 1u ? c.Ny : c.Nz;
+	if(Nax < 3u) return fail(FX_ERR_CONFIG, "a split axis needs at least 3 cells (two halo layers and an interior)");
+	const uint64_t A = face_area(&c, direction);
+	if(A >= 0xFFFFFFFFull) return fail(FX_ERR_CONFIG, "face too large");
+	if(field == FX_FIELD_FI && !fi) return fail(FX_ERR_ARG, "null fi");
+	if(field == FX_FIELD_RHO_U_FLAGS && (!rho || !u || !flags)) return fail(FX_ERR_ARG, "null rho/u/flags");
+	if(field == FX_FIELD_FLAGS && !flags) return fail(FX_ERR_ARG, "null flags");
+	if(field == FX_FIELD_F && !F) return fail(FX_ERR_ARG, "null F");
+	if(field > FX_FIELD_F) return fail(FX_ERR_ARG, "unknown field");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	if(field == FX_FIELD_FI) {
+		ctx->k->transfer_fi(fi, bp, bm, c.Nx, c.Ny, c.Nz, direction, insert, (uint32_t)(t & 1ull), ctx->stream);
+	} else {
+		const unsigned blocks = (unsigned)((A + 255u) / 256u);
+		char* p = (char*)bp; char* m = (char*)bm;
+		if(direction == 0u) { if(insert) k_transfer_field<0, true><<<blocks, 256, 0, ctx->stream>>>(field, p, m, rho, u, flags, F, c.Nx, c.Ny, c.Nz);
+		                      else k_transfer_field<0, false><<<blocks, 256, 0, ctx->stream>>>(field, p, m, rho, u, flags, F, c.Nx, c.Ny, c.Nz); }
+		else if(direction == 1u) { if(insert) k_transfer_field<1, true><<<blocks, 256, 0, ctx->stream>>>(field, p, m, rho, u, flags, F, c.Nx, c.Ny, c.Nz);
+		                           else k_transfer_field<1, false><<<blocks, 256, 0, ctx->stream>>>(field, p, m, rho, u, flags, F, c.Nx, c.Ny, c.Nz); }
+		else { if(insert) k_transfer_field<2, true><<<blocks, 256, 0, ctx->stream>>>(field, p, m, rho, u, flags, F, c.Nx, c.Ny, c.Nz);
+		       else k_transfer_field<2, false><<<blocks, 256, 0, ctx->stream>>>(field, p, m, rho, u, flags, F, c.Nx, c.Ny, c.Nz); }
+	}
+	return check_launch(insert ? "transfer insert" : "transfer extract");
+}
+
+int fx_lbm_transfer_extract(fx_ctx* ctx, uint32_t field, uint32_t direction, uint64_t t, void* buffer_p, void* buffer_m,
+                            const void* fi, const float* rho, const float* u, const uint8_t* flags, const float* F) {
+	return transfer(ctx, field, direction, t, buffer_p, buffer_m, const_cast<void*>(fi), const_cast<float*>(rho), const_cast<float*>(u),
+	                const_cast<uint8_t*>(flags), const_cast<float*>(F), false);
+}
+int fx_lbm_transfer_insert(fx_ctx* ctx, uint32_t field, uint32_t direction, uint64_t t, const void* buffer_p, const void* buffer_m,
+                           void* fi, float* rho, float* u, uint8_t* flags, float* F) {
+	return transfer(ctx, field, direction, t, const_cast<void*>(buffer_p), const_cast<void*>(buffer_m), fi, rho, u, flags, F, true);
 }
 
 int fx_codec_roundtrip(fx_ctx* ctx, uint32_t storage, const float* x, uint16_t* codes, float* decoded, uint64_t n) {

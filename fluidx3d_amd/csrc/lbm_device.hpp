@@ -43,6 +43,7 @@ struct LbmArgs {
 	uint32_t odd;   // parity of the time step t
 	float w;
 	float fx, fy, fz;
+	uint32_t xchunks; // stream_collide on a 1-D XCD-mapped grid: x chunks per row (0: 3-D grid)
 };
 
 // ------------------------------------------------------------------------------------
@@ -329,10 +330,8 @@ __global__ __launch_bounds__(BLOCK) void k_initialize(LbmArgs a) {
 //
 // collide_cell: everything between load_f and store_f for the fluid or TYPE_E cell n;
 // fhn holds the streamed-in DDFs on entry and the post-collision DDFs on return.
-// Writes to rho/u (UPDATE_FIELDS) and moments (RECORD) happen only for live cells; the
-// arithmetic of a dead cell is discarded by the caller.
 template<int Q, bool TRT, uint32_t EXT, bool RECORD>
-__device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint32_t bo, bool live, float* fhn, float* moments) {
+__device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint32_t bo, float* fhn, float* moments) {
 	const bool isE = (a.ext & EXT_EQUILIBRIUM_BOUNDARIES) && bo == TYPE_E;
 	float rhon, uxn, uyn, uzn, rhom1;
 	{
@@ -343,7 +342,7 @@ __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint3
 #pragma unroll
 			for(int i = 1; i < Q; i++) rhom1 += fhn[i];
 		} else rhom1 = rhon - 1.0f;
-		if(RECORD && live) { // pre-collision moments for the adjoint sweep
+		if constexpr(RECORD) { // pre-collision moments for the adjoint sweep
 			moments[n] = rhon; moments[a.N + n] = jx; moments[2ull * a.N + n] = jy; moments[3ull * a.N + n] = jz;
 		}
 		uxn = jx / rhon; uyn = jy / rhon; uzn = jz / rhon;
@@ -363,7 +362,7 @@ __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint3
 #pragma unroll
 		for(int i = 0; i < Q; i++) Fin[i] = 0.0f;
 	}
-	if((a.ext & EXT_UPDATE_FIELDS) && !isE && live) {
+	if((a.ext & EXT_UPDATE_FIELDS) && !isE) {
 		a.rho[n] = rhon; a.u[n] = uxn; a.u[a.N + n] = uyn; a.u[2ull * a.N + n] = uzn;
 	}
 	float feq[Q];
@@ -410,67 +409,59 @@ __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint3
 	}
 }
 
-// C cells per lane: wave v of the block covers the C*64 consecutive cells starting at
-// blockIdx.x*C*blockDim.x + v*C*64, lane l takes cells l, l+64, ..., l+(C-1)*64 of that
-// range.  The loads of all C cells are straight-line code issued before any store (their
-// footprints are disjoint, so the reordering is exact): C*Q loads of one wave in flight,
-// with the row-contiguous pattern of C waves of one cell each.  A lane whose C cells are
-// all solid or outside the row leaves at once; otherwise it also loads (and discards) the
-// DDFs of its dead cells, at clamped in-bounds addresses, so that no branch separates the
-// loads.  Only live cells store.
-template<uint32_t C> __device__ __forceinline__ uint32_t cell_x(uint32_t c) {
-	return blockIdx.x * (blockDim.x * C) + (threadIdx.x & ~63u) * C + c * 64u + (threadIdx.x & 63u);
+// Block -> (x chunk, y, z).  3-D grid: the block indices.  1-D grid (xchunks > 0): the
+// hardware hands consecutive blocks to the 8 XCDs in turn, each with its own L2; the
+// mapping gives every chunk of a row the same XCD (block L: group g = L/(8*xchunks) of 8
+// rows, chunk = (L mod 8*xchunks)/8, row = 8g + L mod 8), so the cache lines that the
+// +-1 x offsets share between neighbouring chunks, stores included, meet in one L2 instead
+// of being written back partially by two.  Returns false for the padding blocks of the
+// last group of 8 rows.
+__device__ __forceinline__ bool block_coords(const LbmArgs& a, uint32_t& chunk, uint32_t& y, uint32_t& z) {
+	if(a.xchunks == 0u) { chunk = blockIdx.x; y = blockIdx.y; z = blockIdx.z; return true; }
+	const uint32_t L = blockIdx.x, span = 8u * a.xchunks;
+	const uint32_t g = L / span, rem = L - g * span;
+	chunk = rem >> 3;
+	const uint32_t row = g * 8u + (rem & 7u);
+	if(row >= a.Ny * a.Nz) return false;
+	z = row / a.Ny;
+	y = row - z * a.Ny;
+	return true;
 }
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD, uint32_t C>
-__device__ __forceinline__ void stream_collide_cells(LbmArgs a, float* moments) {
+// One lane per cell.  All q loads are issued before any of them is decoded or used (the
+// scheduler, left alone, decodes each 16-bit value right after its load and waits for it,
+// one load at a time).
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD>
+__device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 	a.ext &= EXT;
+	uint32_t chunk, y, z;
+	if(!block_coords(a, chunk, y, z)) return;
+	const uint32_t x = chunk * blockDim.x + threadIdx.x;
+	if(x >= a.Nx) return;
+	const Site s = make_site(x, y, z, a.Nx, a.Ny, a.Nz);
+	const uint64_t n = s.n();
+	const uint32_t flagsn = a.flags[n];
+	const uint32_t bo = flagsn & TYPE_BO;
+	if(bo == TYPE_S || (flagsn & TYPE_SU) == TYPE_G) return; // solid cells are mailboxes
 	typedef typename Storage<ST>::E E;
 	E* fi = (E*)a.fi;
-	Site s[C];
-	uint32_t bo[C];
-	bool live[C];
-	uint32_t flagsn[C];
-#pragma unroll
-	for(uint32_t c = 0u; c < C; c++) {
-		const uint32_t x = cell_x<C>(c);
-		live[c] = x < a.Nx;
-		s[c] = make_site(live[c] ? x : a.Nx - 1u, blockIdx.y, blockIdx.z, a.Nx, a.Ny, a.Nz);
-	}
-#pragma unroll
-	for(uint32_t c = 0u; c < C; c++) flagsn[c] = a.flags[s[c].n()];
-	bool any = false;
-#pragma unroll
-	for(uint32_t c = 0u; c < C; c++) {
-		bo[c] = flagsn[c] & TYPE_BO;
-		live[c] = live[c] && !(bo[c] == TYPE_S || (flagsn[c] & TYPE_SU) == TYPE_G); // solid cells are mailboxes
-		any = any || live[c];
-	}
-	if(!any) return;
-	typename Storage<ST>::E r[C][Q];
-#pragma unroll
-	for(uint32_t c = 0u; c < C; c++) load_raw<Q, ST>(fi, a.N, s[c], a.odd, r[c]);
-	// Keep every load above this point: left alone, the scheduler may decode each 16-bit
-	// value right after its load (to save registers) and wait for it, one load at a time.
+	E r[Q];
+	load_raw<Q, ST>(fi, a.N, s, a.odd, r);
 	__builtin_amdgcn_sched_barrier(0);
-	float f[C][Q];
+	float f[Q];
 #pragma unroll
-	for(uint32_t c = 0u; c < C; c++)
-#pragma unroll
-		for(int i = 0; i < Q; i++) f[c][i] = Storage<ST>::decode(r[c][i]);
-#pragma unroll
-	for(uint32_t c = 0u; c < C; c++) collide_cell<Q, TRT, EXT, RECORD>(a, s[c].n(), bo[c], live[c], f[c], moments);
-#pragma unroll
-	for(uint32_t c = 0u; c < C; c++) if(live[c]) store_f<Q, ST>(fi, a.N, s[c], a.odd, f[c]);
+	for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
+	collide_cell<Q, TRT, EXT, RECORD>(a, n, bo, f, moments);
+	store_f<Q, ST>(fi, a.N, s, a.odd, f);
 }
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, uint32_t C = 1u>
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
-	stream_collide_cells<Q, ST, TRT, EXT, false, C>(a, nullptr);
+	stream_collide_cell<Q, ST, TRT, EXT, false>(a, nullptr);
 }
 // stream_collide that also writes the pre-collision (rho, jx, jy, jz) of every fluid cell to
 // moments[c*N+n], c = 0..3 (16 B/cell): what the adjoint step needs of the forward state.
 template<int Q, uint32_t ST, bool TRT>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide_record(LbmArgs a, float* moments) {
-	stream_collide_cells<Q, ST, TRT, EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY | EXT_PRECISE_DENSITY, true, 1u>(a, moments);
+	stream_collide_cell<Q, ST, TRT, EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY | EXT_PRECISE_DENSITY, true>(a, moments);
 }
 
 // update_fields (src/kernel.cpp:1794-1870)
@@ -537,6 +528,80 @@ __global__ __launch_bounds__(BLOCK) void k_mass_partial(LbmArgs a, double* parti
 		__syncthreads();
 	}
 	if(threadIdx.x == 0) partial[((uint64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// ------------------------------------------------------------------------------------
+// Halo transfer between domains of a split lattice (the role of src/kernel.cpp:2049-2124,
+// transfer_extract_fi / transfer__insert_fi, and 2126-2183 for rho/u/flags, flags, F).  A
+// domain split along an axis has one halo layer on each split side; after every
+// stream_collide, the DDF values that cross each face move to the neighbour:
+//  - extract side +: interior layer N-2, the directions with c_axis(i) = +1;
+//  - extract side -: interior layer 1, the directions with c_axis(i) = -1;
+//  - insert side +: halo layer N-1, the directions with c_axis(i) = -1 (what the + neighbour
+//    extracted on its side -);
+//  - insert side -: halo layer 0, the directions with c_axis(i) = +1.
+// For cell n and direction i the value is the population f_i that stream_collide of step t
+// stored: at (n + e_i, slot t odd ? i+1 : i) for odd i, at (n, slot t odd ? i-1 : i) for
+// even i (store_f).  Extract reads it relative to the interior cell, insert writes the same
+// physical location relative to the receiving domain's halo cell.  The values are copied as
+// stored (no decode).  Buffer layout b*A + a: a = face cell (x faces: y + z*Ny, y faces:
+// x + z*Nx, z faces: x + y*Nx), b = position of i in the increasing direction list.
+// ------------------------------------------------------------------------------------
+template<int Q> struct TransferList {
+	static constexpr int T = Q == 19 ? 5 : 9; // directions crossing one face (src/lbm.cpp:18,22)
+	int8_t i[3][2][9];                        // [axis][side: 0 = +, 1 = -][b]
+	constexpr TransferList() : i{} {
+		for(int ax = 0; ax < 3; ax++)
+			for(int sd = 0; sd < 2; sd++) {
+				int b = 0;
+				for(int k = 1; k < Q; k++) {
+					const int c = ax == 0 ? VS<Q>::cx[k] : ax == 1 ? VS<Q>::cy[k] : VS<Q>::cz[k];
+					if(c == (sd == 0 ? 1 : -1)) i[ax][sd][b++] = (int8_t)k;
+				}
+			}
+	}
+};
+__device__ __forceinline__ uint32_t wrap(int v, uint32_t n) { return v < 0 ? n - 1u : (uint32_t)v >= n ? 0u : (uint32_t)v; }
+// Face cell a of the layer `layer` normal to AXIS -> (x, y, z)
+template<int AXIS> __device__ __forceinline__ void face_cell(uint32_t a, uint32_t layer, uint32_t Nx, uint32_t Ny, uint32_t& x, uint32_t& y, uint32_t& z) {
+	if constexpr(AXIS == 0) { x = layer; y = a % Ny; z = a / Ny; }
+	else if constexpr(AXIS == 1) { x = a % Nx; y = layer; z = a / Nx; }
+	else { x = a % Nx; y = a / Nx; z = layer; }
+}
+// DDF location of direction i stored by cell (x,y,z) at parity odd
+template<int Q> __device__ __forceinline__ uint64_t transfer_location(int i, uint32_t x, uint32_t y, uint32_t z, uint32_t Nx, uint32_t Ny, uint32_t Nz, uint64_t N, uint32_t odd) {
+	uint64_t slot;
+	if(i & 1) {
+		x = wrap((int)x + VS<Q>::cx[i], Nx); y = wrap((int)y + VS<Q>::cy[i], Ny); z = wrap((int)z + VS<Q>::cz[i], Nz);
+		slot = odd ? (uint64_t)(i + 1) : (uint64_t)i;
+	} else {
+		slot = odd ? (uint64_t)(i - 1) : (uint64_t)i;
+	}
+	return slot * N + ((uint64_t)z * Ny + y) * Nx + x;
+}
+template<int Q, typename E, int AXIS, bool INSERT>
+__global__ __launch_bounds__(BLOCK) void k_transfer_fi(E* fi, E* buffer_p, E* buffer_m, uint32_t Nx, uint32_t Ny, uint32_t Nz, uint32_t odd) {
+	constexpr TransferList<Q> L{};
+	constexpr int T = TransferList<Q>::T;
+	const uint32_t Nax = AXIS == 0 ? Nx : AXIS == 1 ? Ny : Nz;
+	const uint64_t A = AXIS == 0 ? (uint64_t)Ny * Nz : AXIS == 1 ? (uint64_t)Nx * Nz : (uint64_t)Nx * Ny;
+	const uint64_t a = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if(a >= A) return;
+	const uint64_t N = (uint64_t)Nx * Ny * Nz;
+#pragma unroll
+	for(int sd = 0; sd < 2; sd++) { // side +, side -
+		E* buffer = sd == 0 ? buffer_p : buffer_m;
+		const uint32_t layer = INSERT ? (sd == 0 ? Nax - 1u : 0u) : (sd == 0 ? Nax - 2u : 1u);
+		uint32_t x, y, z;
+		face_cell<AXIS>((uint32_t)a, layer, Nx, Ny, x, y, z);
+#pragma unroll
+		for(int b = 0; b < T; b++) {
+			const int i = L.i[AXIS][INSERT ? 1 - sd : sd][b];
+			const uint64_t m = transfer_location<Q>(i, x, y, z, Nx, Ny, Nz, N, odd);
+			if constexpr(INSERT) fi[m] = buffer[(uint64_t)b * A + a];
+			else buffer[(uint64_t)b * A + a] = fi[m];
+		}
+	}
 }
 
 // ------------------------------------------------------------------------------------

@@ -7,16 +7,32 @@ namespace fx {
 
 struct LaunchShape {
 	dim3 grid, block;
-	size_t lds = 0; // dynamic LDS reserved per block (occupancy limit), see occupancy_lds()
+	size_t lds = 0;       // dynamic LDS reserved per block (occupancy limit), see occupancy_lds()
+	uint32_t xchunks = 0; // 1-D XCD-mapped grid with this many x chunks per row (see block_coords)
 };
-// Block of bx lanes (bx = 64, 128 or 256) over bx*cells consecutive x cells of one row;
-// grid over (x chunk, y, z).  bx is the smallest width that covers the row, at most 256.
-inline LaunchShape lattice_shape(uint32_t Nx, uint32_t Ny, uint32_t Nz, uint32_t cells = 1u) {
-	const uint32_t lanes = (Nx + cells - 1u) / cells;
-	const uint32_t bx = lanes <= 64u ? 64u : lanes <= 128u ? 128u : 256u;
+// Block of bx consecutive x cells of one row (bx = 64, 128 or 256); grid over (x chunk,
+// y, z).  bx is the widest block that leaves less than one idle wave per row (64 when none
+// does).
+inline LaunchShape lattice_shape(uint32_t Nx, uint32_t Ny, uint32_t Nz) {
+	uint32_t bx = 64u;
+	for(uint32_t b = 256u; b > 64u; b >>= 1)
+		if((Nx + b - 1u) / b * b - Nx < 64u) { bx = b; break; }
 	LaunchShape s;
 	s.block = dim3(bx, 1, 1);
-	s.grid = dim3((Nx + bx * cells - 1u) / (bx * cells), Ny, Nz);
+	s.grid = dim3((Nx + bx - 1u) / bx, Ny, Nz);
+	return s;
+}
+
+// stream_collide: with xcd_map, rows that take more than one block use the 1-D XCD-mapped
+// grid, which keeps all chunks of a row on one XCD (block_coords in lbm_device.hpp).
+inline LaunchShape stream_shape(uint32_t Nx, uint32_t Ny, uint32_t Nz, bool xcd_map) {
+	LaunchShape s = lattice_shape(Nx, Ny, Nz);
+	const uint32_t chunks = s.grid.x;
+	const uint64_t groups = ((uint64_t)Ny * Nz + 7u) / 8u;
+	if(xcd_map && chunks > 1u && groups * 8u * chunks < 0x7FFFFFFFull) {
+		s.grid = dim3((unsigned)(groups * 8u * chunks), 1, 1);
+		s.xchunks = chunks;
+	}
 	return s;
 }
 
@@ -36,6 +52,8 @@ typedef void (*mass_fn)(const LbmArgs&, const LaunchShape&, hipStream_t, double*
 typedef void (*record_fn)(const LbmArgs&, const LaunchShape&, hipStream_t, float*);
 typedef void (*adjoint_fn)(const AdjointArgs&, const LaunchShape&, hipStream_t);
 typedef void (*objective_fn)(const LbmArgs&, uint32_t, hipStream_t, double*);
+typedef void (*transfer_fn)(void* fi, void* buffer_p, void* buffer_m, uint32_t Nx, uint32_t Ny, uint32_t Nz, uint32_t axis, bool insert,
+                            uint32_t odd, hipStream_t);
 
 struct LbmKernels {
 	lattice_fn initialize;
@@ -47,26 +65,36 @@ struct LbmKernels {
 	record_fn stream_collide_record;
 	adjoint_fn adjoint_step;
 	objective_fn objective_partial; // grid of ceil(Ny*Nz/256) blocks, one partial each
-	uint32_t stream_collide_cells;  // cells per lane of stream_collide (its launch shape)
+	bool stream_collide_xcd_map;    // 1-D XCD-mapped grid for rows of several blocks
+	transfer_fn transfer_fi;        // halo transfer of the DDFs (k_transfer_fi)
 };
 
 constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY | EXT_PRECISE_DENSITY;
 
-// Measured launch tuning (see KernelSet).  The plain FP32 kernel streams fastest with
-// fewer concurrent DDF streams (3 waves/SIMD); the 2-byte formats need full occupancy;
-// D3Q19 FP16C, the most ALU-heavy variant, gains from two cells per lane.
-constexpr uint32_t tune_cells(int Q, uint32_t ST, uint32_t EXT) {
-	return (ST == ST_FP16C && Q == 19) ? 2u : 1u;
-}
+// Measured launch tuning (see KernelSet), MI355X, 256^3 to 1536x768x768:
+// - the plain FP32 kernel (non-temporal DDF access) streams fastest with fewer concurrent
+//   DDF streams, 3 waves/SIMD, on the 3-D grid;
+// - the 2-byte formats need full occupancy;
+// - FP16S gains 9-24 % on rows of several blocks from the XCD-mapped grid; FP32 loses up
+//   to 10 % there and FP16C, bound by its codec arithmetic, is indifferent;
+// - two or four cells per lane measured no better than one for any variant once all
+//   loads are issued ahead of the arithmetic.
 constexpr uint32_t tune_waves_cap(int Q, uint32_t ST, uint32_t EXT) {
 	return (EXT == 0u && ST == ST_FP32) ? 3u : 0u;
+}
+constexpr bool tune_xcd_map(int Q, uint32_t ST, uint32_t EXT) {
+	return ST == ST_FP16S;
 }
 
 // Kernel set of one (velocity set, storage, collision) variant; EXT = 0 instantiates the
 // extension-free kernels, EXT_ALL the ones that read the runtime extension mask.
 template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static void initialize(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_initialize<Q, ST><<<s.grid, s.block, 0, st>>>(a); }
-	static void stream_collide(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_stream_collide<Q, ST, TRT, EXT, cells><<<s.grid, s.block, s.lds, st>>>(a); }
+	static void stream_collide(const LbmArgs& a, const LaunchShape& s, hipStream_t st) {
+		LbmArgs b = a;
+		b.xchunks = s.xchunks;
+		k_stream_collide<Q, ST, TRT, EXT><<<s.grid, s.block, s.lds, st>>>(b);
+	}
 	static void update_fields(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_fields<Q, ST, (EXT == 0u ? 0u : EXT_ALL)><<<s.grid, s.block, 0, st>>>(a); }
 	static void update_force_field(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_force_field<Q, ST><<<s.grid, s.block, 0, st>>>(a); }
 	static void mass_partial(const LbmArgs& a, const LaunchShape& s, hipStream_t st, double* p) { k_mass_partial<Q, ST><<<s.grid, s.block, 0, st>>>(a, p); }
@@ -76,14 +104,24 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 		const uint64_t cells = (uint64_t)a.Ny * a.Nz;
 		k_objective_partial<Q, ST><<<(unsigned)((cells + 255u) / 256u), 256, 0, st>>>(a, plane_x, p);
 	}
-	// Launch tuning of stream_collide, measured on MI355X at 256^3 (D3Q19 and D3Q27, SRT,
-	// every storage; sweep of cells per lane x block width x residency cap):
-	// cells per lane, and a residency cap in waves per SIMD (0 = none).
-	static constexpr uint32_t cells = tune_cells(Q, ST, EXT);
+	// Launch tuning of stream_collide: residency cap in waves per SIMD (0 = none) and grid.
 	static constexpr uint32_t waves_cap = tune_waves_cap(Q, ST, EXT);
+	static constexpr bool xcd_map = tune_xcd_map(Q, ST, EXT);
+	template<int AXIS> static void transfer_axis(void* fi, void* bp, void* bm, uint32_t Nx, uint32_t Ny, uint32_t Nz, bool insert, uint32_t odd, hipStream_t st) {
+		typedef typename Storage<ST>::E E;
+		const uint64_t A = AXIS == 0 ? (uint64_t)Ny * Nz : AXIS == 1 ? (uint64_t)Nx * Nz : (uint64_t)Nx * Ny;
+		const unsigned blocks = (unsigned)((A + 255u) / 256u);
+		if(insert) k_transfer_fi<Q, E, AXIS, true><<<blocks, 256, 0, st>>>((E*)fi, (E*)bp, (E*)bm, Nx, Ny, Nz, odd);
+		else k_transfer_fi<Q, E, AXIS, false><<<blocks, 256, 0, st>>>((E*)fi, (E*)bp, (E*)bm, Nx, Ny, Nz, odd);
+	}
+	static void transfer_fi(void* fi, void* bp, void* bm, uint32_t Nx, uint32_t Ny, uint32_t Nz, uint32_t axis, bool insert, uint32_t odd, hipStream_t st) {
+		if(axis == 0u) transfer_axis<0>(fi, bp, bm, Nx, Ny, Nz, insert, odd, st);
+		else if(axis == 1u) transfer_axis<1>(fi, bp, bm, Nx, Ny, Nz, insert, odd, st);
+		else transfer_axis<2>(fi, bp, bm, Nx, Ny, Nz, insert, odd, st);
+	}
 	static const LbmKernels* table() {
 		static const LbmKernels k = {initialize, stream_collide, update_fields, update_force_field, mass_partial, waves_cap,
-		                             stream_collide_record, adjoint_step, objective_partial, cells};
+		                             stream_collide_record, adjoint_step, objective_partial, xcd_map, transfer_fi};
 		return &k;
 	}
 };

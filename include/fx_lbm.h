@@ -228,6 +228,31 @@ int fx_lbm_adjoint_step(fx_ctx* ctx, float* lambda, const float* rho, const floa
 /* plane x-momentum of the populations loaded at step t (FP64 sum), synchronous */
 int fx_lbm_objective_plane(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t t, uint32_t plane_x, double* out);
 
+/* ---- halo transfer between the domains of a split lattice ----
+ *
+ * A lattice split into domains along an axis gives every domain one halo layer on each
+ * split side (src/lbm.cpp:721-733: domain size N/D + 2 along a split axis).  After each
+ * stream_collide at step t, fx_lbm_transfer_extract of FX_FIELD_FI packs the DDF values
+ * that cross the two faces normal to `direction` (0 = x, 1 = y, 2 = z) into buffer_p
+ * (+ face) and buffer_m (- face); the neighbours then fx_lbm_transfer_insert them:
+ * domain d inserts as buffer_p the buffer_m of its + neighbour and as buffer_m the
+ * buffer_p of its - neighbour.  Directions are communicated one after another (x, y, z),
+ * each extract finished before the matching insert.  Field data (FX_FIELD_RHO_U_FLAGS:
+ * 17 B per face cell, FX_FIELD_FLAGS: 1 B, FX_FIELD_F: 12 B) moves from the interior
+ * layers 1 and N-2 into the halo layers 0 and N-1 the same way.  Buffers are device
+ * memory of fx_transfer_bytes each; values move as stored (no FP16 decode).  Replace the
+ * transfer kernels of src/kernel.cpp:2049-2183 and LBM_Domain::enqueue_transfer_*_field
+ * (src/lbm.cpp:1343-1354); the exchange itself (LBM::communicate_field,
+ * src/lbm.cpp:1355-1383, host pointer swaps over PCIe) is the caller's, device to
+ * device. */
+enum { FX_FIELD_FI = 0, FX_FIELD_RHO_U_FLAGS = 1, FX_FIELD_FLAGS = 2, FX_FIELD_F = 3 };
+/* bytes of one side's buffer for `field` across the faces normal to `direction` */
+uint64_t fx_transfer_bytes(const fx_lbm_config* cfg, uint32_t field, uint32_t direction);
+int fx_lbm_transfer_extract(fx_ctx* ctx, uint32_t field, uint32_t direction, uint64_t t, void* buffer_p, void* buffer_m,
+                            const void* fi, const float* rho, const float* u, const uint8_t* flags, const float* F);
+int fx_lbm_transfer_insert(fx_ctx* ctx, uint32_t field, uint32_t direction, uint64_t t, const void* buffer_p, const void* buffer_m,
+                           void* fi, float* rho, float* u, uint8_t* flags, float* F);
+
 /* ---- diagnostics ---- */
 
 /* Storage codec round trip on the device: codes[k] = encode(x[k]), decoded[k] =

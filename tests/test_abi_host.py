@@ -76,3 +76,14 @@ def test_relaxation_rate_known_values(oracle):
     assert fx.relaxation_rate(0.048) == np.float32(1.55279504)
     for nu in (0.01, 0.2, 1.0, 3.3e-3):
         assert fx.relaxation_rate(nu) == oracle.relaxation_rate(nu)
+
+
+def test_transfer_buffer_sizes():
+    """fx_transfer_bytes: transfers x face area x bytes (src/lbm.cpp:1385-1397)."""
+    from fluidx3d_amd import _abi
+    L = _abi.load()
+    for q, T in ((19, 5), (27, 9)):
+        for storage, b in (("FP32", 4), ("FP16S", 2)):
+            c = _abi.make_config(10, 6, 4, 1.0, q, "SRT", storage, 0)
+            assert [L.fx_transfer_bytes(c, 0, a) for a in range(3)] == [6 * 4 * T * b, 10 * 4 * T * b, 10 * 6 * T * b]
+            assert [L.fx_transfer_bytes(c, f, 0) for f in (1, 2, 3)] == [24 * 17, 24, 24 * 12]
