@@ -416,8 +416,8 @@ __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint3
 // +-1 x offsets share between neighbouring chunks, stores included, meet in one L2 instead
 // of being written back partially by two.  Returns false for the padding blocks of the
 // last group of 8 rows.
-__device__ __forceinline__ bool block_coords(const LbmArgs& a, uint32_t& chunk, uint32_t& y, uint32_t& z) {
-	if(a.xchunks == 0u) { chunk = blockIdx.x; y = blockIdx.y; z = blockIdx.z; return true; }
+template<bool XCD> __device__ __forceinline__ bool block_coords(const LbmArgs& a, uint32_t& chunk, uint32_t& y, uint32_t& z) {
+	if(!XCD || a.xchunks == 0u) { chunk = blockIdx.x; y = blockIdx.y; z = blockIdx.z; return true; }
 	const uint32_t L = blockIdx.x, span = 8u * a.xchunks;
 	const uint32_t g = L / span, rem = L - g * span;
 	chunk = rem >> 3;
@@ -427,14 +427,12 @@ __device__ __forceinline__ bool block_coords(const LbmArgs& a, uint32_t& chunk, 
 	y = row - z * a.Ny;
 	return true;
 }
-// One lane per cell.  All q loads are issued before any of them is decoded or used (the
-// scheduler, left alone, decodes each 16-bit value right after its load and waits for it,
-// one load at a time).
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD>
+// One lane per cell.
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD, bool XCD>
 __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 	a.ext &= EXT;
 	uint32_t chunk, y, z;
-	if(!block_coords(a, chunk, y, z)) return;
+	if(!block_coords<XCD>(a, chunk, y, z)) return;
 	const uint32_t x = chunk * blockDim.x + threadIdx.x;
 	if(x >= a.Nx) return;
 	const Site s = make_site(x, y, z, a.Nx, a.Ny, a.Nz);
@@ -444,24 +442,32 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 	if(bo == TYPE_S || (flagsn & TYPE_SU) == TYPE_G) return; // solid cells are mailboxes
 	typedef typename Storage<ST>::E E;
 	E* fi = (E*)a.fi;
-	E r[Q];
-	load_raw<Q, ST>(fi, a.N, s, a.odd, r);
-	__builtin_amdgcn_sched_barrier(0);
 	float f[Q];
+	if constexpr(ST == ST_FP16S) {
+		// every load is issued before the first decode: left alone, the scheduler may
+		// convert each 16-bit value right after its load and wait for it, one at a time
+		E r[Q];
+		load_raw<Q, ST>(fi, a.N, s, a.odd, r);
+		__builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-	for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
+		for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
+	} else {
+		load_f<Q, ST>(fi, a.N, s, a.odd, f);
+	}
 	collide_cell<Q, TRT, EXT, RECORD>(a, n, bo, f, moments);
 	store_f<Q, ST>(fi, a.N, s, a.odd, f);
 }
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT>
+// XCD: the kernel may run on the 1-D XCD-mapped grid (a.xchunks > 0); false compiles the
+// mapping out.
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
-	stream_collide_cell<Q, ST, TRT, EXT, false>(a, nullptr);
+	stream_collide_cell<Q, ST, TRT, EXT, false, XCD>(a, nullptr);
 }
 // stream_collide that also writes the pre-collision (rho, jx, jy, jz) of every fluid cell to
 // moments[c*N+n], c = 0..3 (16 B/cell): what the adjoint step needs of the forward state.
 template<int Q, uint32_t ST, bool TRT>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide_record(LbmArgs a, float* moments) {
-	stream_collide_cell<Q, ST, TRT, EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY | EXT_PRECISE_DENSITY, true>(a, moments);
+	stream_collide_cell<Q, ST, TRT, EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY | EXT_PRECISE_DENSITY, true, false>(a, moments);
 }
 
 // update_fields (src/kernel.cpp:1794-1870)
