@@ -149,9 +149,24 @@ def measured_copy_peak(fx):
     return 2.0 * nbytes * 10 / (ms.value * 1e-3) * 1e-9
 
 
+def cpu_opencl_devices():
+    """CPU OpenCL devices visible to clinfo (the reference's CPU path needs one)."""
+    import shutil
+    import subprocess
+    if not shutil.which("clinfo"):
+        return "clinfo not installed"
+    try:
+        out = subprocess.run(["clinfo"], capture_output=True, text=True, timeout=60).stdout
+    except Exception as e:  # noqa: BLE001
+        return f"clinfo failed: {e}"
+    cpus = [l for l in out.splitlines() if "Device Type" in l and "CPU" in l]
+    return f"{len(cpus)} CPU OpenCL device(s)"
+
+
 def cpu_baseline(size, seconds):
-    """The oracle's FP32 restatement (a port, not the reference: no CPU OpenCL runtime is
-    installed on the GPU host), multithreaded, timed on a bounded number of steps."""
+    """The oracle's FP32 restatement (a port, not the reference: the reference's CPU path
+    needs a CPU OpenCL runtime, which the GPU host lacks), multithreaded, timed on a
+    bounded number of steps."""
     from oracle import oracle as O
     from tests import scenes
     threads = min(16, os.cpu_count() or 1)
@@ -172,7 +187,8 @@ def cpu_baseline(size, seconds):
     return {"value": ref.N * steps / el * 1e-6, "unit": "MLUPs/s", "cores": threads, "kind": "port",
             "sample": f"{size}^3 D3Q19 SRT FP32 Taylor-Green, {steps} steps after 1 warm-up step, oracle FP32 "
                       f"restatement on {threads} threads of {platform.processor() or platform.machine()} "
-                      f"({os.cpu_count()} logical CPUs); reference CPU OpenCL baseline unavailable (no CPU OpenCL runtime)"}
+                      f"({os.cpu_count()} logical CPUs); reference CPU OpenCL baseline unavailable "
+                      f"(clinfo: {cpu_opencl_devices()}; the reference source is not shipped to the GPU host)"}
 
 
 def main():

@@ -19,13 +19,16 @@ FX_EQUILIBRIUM_BOUNDARIES = 4
 FX_UPDATE_FIELDS = 8
 FX_POROSITY = 16
 FX_PRECISE_DENSITY = 32
+FX_MOVING_BOUNDARIES = 64
+FX_SUBGRID = 128
 FX_FIELD_FI, FX_FIELD_RHO_U_FLAGS, FX_FIELD_FLAGS, FX_FIELD_F = 0, 1, 2, 3
 
 STORAGE = {"FP32": FX_FP32, "FP16S": FX_FP16S, "FP16C": FX_FP16C}
 COLLISION = {"SRT": FX_SRT, "TRT": FX_TRT}
 EXTENSIONS = {"VOLUME_FORCE": FX_VOLUME_FORCE, "FORCE_FIELD": FX_FORCE_FIELD,
               "EQUILIBRIUM_BOUNDARIES": FX_EQUILIBRIUM_BOUNDARIES, "UPDATE_FIELDS": FX_UPDATE_FIELDS,
-              "POROSITY": FX_POROSITY, "PRECISE_DENSITY": FX_PRECISE_DENSITY}
+              "POROSITY": FX_POROSITY, "PRECISE_DENSITY": FX_PRECISE_DENSITY,
+              "MOVING_BOUNDARIES": FX_MOVING_BOUNDARIES, "SUBGRID": FX_SUBGRID}
 
 # every symbol include/fx_lbm.h declares
 EXPORTS = [
@@ -37,6 +40,7 @@ EXPORTS = [
     "fx_lbm_mass_sum", "fx_codec_roundtrip", "fx_bench_copy", "fx_adjoint_memory", "fx_adjoint_create",
     "fx_adjoint_destroy", "fx_lbm_forward", "fx_lbm_backward", "fx_lbm_stream_collide_record", "fx_lbm_adjoint_step",
     "fx_lbm_objective_plane", "fx_transfer_bytes", "fx_lbm_transfer_extract", "fx_lbm_transfer_insert",
+    "fx_lbm_update_moving_boundaries",
 ]
 
 
@@ -95,6 +99,7 @@ def load():
     L.fx_buf_fill.argtypes = [vp, vp, u64, ctypes.c_uint8]
     L.fx_buf_copy.argtypes = [vp, vp, vp, u64]
     L.fx_lbm_initialize.argtypes = [vp, vp, vp, vp, vp]
+    L.fx_lbm_update_moving_boundaries.argtypes = [vp, vp, vp]
     step = [vp, vp, vp, vp, vp, u64, f32, f32, f32, vp, vp]
     L.fx_lbm_stream_collide.argtypes = step
     L.fx_lbm_update_fields.argtypes = step

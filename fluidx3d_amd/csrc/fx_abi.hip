@@ -27,7 +27,8 @@ int fail_hip(hipError_t e, const char* what) {
 
 bool valid_enums(const fx_lbm_config* c) {
 	return c && (c->velocity_set == FX_D3Q19 || c->velocity_set == FX_D3Q27) && c->collision <= FX_TRT && c->storage <= FX_FP16C &&
-	       (c->extensions & ~(FX_VOLUME_FORCE | FX_FORCE_FIELD | FX_EQUILIBRIUM_BOUNDARIES | FX_UPDATE_FIELDS | FX_POROSITY | FX_PRECISE_DENSITY)) == 0u;
+	       (c->extensions & ~(FX_VOLUME_FORCE | FX_FORCE_FIELD | FX_EQUILIBRIUM_BOUNDARIES | FX_UPDATE_FIELDS | FX_POROSITY | FX_PRECISE_DENSITY |
+	                          FX_MOVING_BOUNDARIES | FX_SUBGRID)) == 0u;
 }
 
 } // namespace
@@ -336,6 +337,7 @@ uint64_t fx_bandwidth_bytes_per_cell(const fx_lbm_config* c) {
 	if(c->extensions & FX_UPDATE_FIELDS) b += 16u;
 	if(c->extensions & FX_FORCE_FIELD) b += 12u;
 	if(c->extensions & FX_POROSITY) b += 4u;
+	if(c->extensions & FX_MOVING_BOUNDARIES) b += (uint64_t)c->velocity_set - 1u; // neighbour flags (src/lbm.cpp:62-64)
 	return b;
 }
 
@@ -462,13 +464,23 @@ int fx_buf_copy(fx_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
 	return FX_OK;
 }
 
-int fx_lbm_initialize(fx_ctx* ctx, void* fi, const float* rho, float* u, const uint8_t* flags) {
+int fx_lbm_initialize(fx_ctx* ctx, void* fi, const float* rho, float* u, uint8_t* flags) {
 	if(!ctx) return fail(FX_ERR_ARG, "null context");
 	if(!fi || !rho || !u || !flags) return fail(FX_ERR_ARG, "fi, rho, u and flags must be device pointers");
 	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
 	const fx::LbmArgs a = make_args(ctx, fi, rho, u, flags, 1ull, 0.0f, 0.0f, 0.0f, nullptr, nullptr);
 	ctx->k->initialize(a, ctx->shape, ctx->stream);
 	return check_launch("initialize");
+}
+
+int fx_lbm_update_moving_boundaries(fx_ctx* ctx, const float* u, uint8_t* flags) {
+	if(!ctx) return fail(FX_ERR_ARG, "null context");
+	if(!u || !flags) return fail(FX_ERR_ARG, "u and flags must be device pointers");
+	if(!(ctx->cfg.extensions & FX_MOVING_BOUNDARIES)) return fail(FX_ERR_CONFIG, "MOVING_BOUNDARIES is not enabled");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	const fx::LbmArgs a = make_args(ctx, nullptr, nullptr, u, flags, 0ull, 0.0f, 0.0f, 0.0f, nullptr, nullptr);
+	ctx->k->update_moving_boundaries(a, ctx->shape, ctx->stream);
+	return check_launch("update_moving_boundaries");
 }
 
 int fx_lbm_stream_collide(fx_ctx* ctx, void* fi, float* rho, float* u, const uint8_t* flags, uint64_t t, float fx_, float fy_, float fz_,
@@ -577,6 +589,8 @@ static uint64_t ddf_bytes(const fx_ctx* ctx) {
 }
 static int adjoint_layout(const fx_ctx* ctx, const fx_adjoint_config* cfg, uint64_t& k, uint64_t& checkpoints, uint64_t& bytes) {
 	if(!ctx || !cfg) return fail(FX_ERR_ARG, "null argument");
+	if(ctx->cfg.extensions & (FX_MOVING_BOUNDARIES | FX_SUBGRID))
+		return fail(FX_ERR_CONFIG, "the adjoint does not differentiate MOVING_BOUNDARIES or SUBGRID");
 	if(cfg->steps == 0u || cfg->window_begin >= cfg->steps) return fail(FX_ERR_ARG, "need 0 <= window_begin < steps");
 	if(cfg->plane_x >= ctx->cfg.Nx) return fail(FX_ERR_ARG, "plane_x out of range");
 	k = cfg->checkpoint_interval;

@@ -22,12 +22,14 @@ namespace fx {
 
 enum : uint32_t {
 	EXT_VOLUME_FORCE = 1u, EXT_FORCE_FIELD = 2u, EXT_EQUILIBRIUM_BOUNDARIES = 4u,
-	EXT_UPDATE_FIELDS = 8u, EXT_POROSITY = 16u, EXT_PRECISE_DENSITY = 32u
+	EXT_UPDATE_FIELDS = 8u, EXT_POROSITY = 16u, EXT_PRECISE_DENSITY = 32u, EXT_MOVING_BOUNDARIES = 64u,
+	EXT_SUBGRID = 128u
 };
 enum : uint32_t { ST_FP32 = 0u, ST_FP16S = 1u, ST_FP16C = 2u };
-constexpr uint32_t TYPE_S = 0x01u, TYPE_E = 0x02u, TYPE_BO = 0x03u, TYPE_SU = 0x38u, TYPE_G = 0x20u;
+constexpr uint32_t TYPE_S = 0x01u, TYPE_E = 0x02u, TYPE_MS = 0x03u, TYPE_BO = 0x03u, TYPE_T = 0x04u, TYPE_SU = 0x38u, TYPE_G = 0x20u;
 constexpr float DEF_C = 0.57735027f; // lattice speed of sound, src/lbm.cpp:366
 constexpr float THIRD = 0.33333334f; // src/kernel.cpp:1093-1099
+constexpr float SMAGORINSKY = 0.76421222f; // 18*sqrt(2)*(C*Delta)^2, src/kernel.cpp:1591
 
 // Kernel arguments, passed by value (kernarg segment -> scalar registers).
 struct LbmArgs {
@@ -308,6 +310,26 @@ __device__ __forceinline__ void store_f(typename Storage<ST>::E* fi, uint64_t N,
 constexpr int BLOCK = 256;
 
 // initialize (src/kernel.cpp:1358-1430), non-moving boundaries
+// neighbour cell (dx,dy,dz) of a site
+template<int Q> __device__ __forceinline__ uint64_t neighbour(const Site& s, int i) {
+	return s.row(VS<Q>::cy[i], VS<Q>::cz[i]) + s.X[VS<Q>::cx[i] + 1];
+}
+// true if a neighbour of the cell is a solid cell with non-zero velocity (src/kernel.cpp
+// :1395-1398, 1441-1444)
+template<int Q> __device__ __forceinline__ bool next_to_moving_solid(const LbmArgs& a, const Site& s) {
+	bool m = false;
+#pragma unroll
+	for(int i = 1; i < Q; i++) {
+		const uint64_t j = neighbour<Q>(s, i);
+		m = m || ((a.flags[j] & TYPE_BO) == TYPE_S && (a.u[j] != 0.0f || a.u[a.N + j] != 0.0f || a.u[2ull * a.N + j] != 0.0f));
+	}
+	return m;
+}
+// initialize (src/kernel.cpp:1358-1430), no SURFACE/TEMPERATURE.  Without MOVING_BOUNDARIES
+// every solid cell's velocity is zeroed; with it only that of solid cells whose neighbours
+// are all solid, and every other non-TYPE_E cell is marked (or unmarked) TYPE_MS.  A cell
+// writes only its own flags and velocity, and the marks never change what the other cells
+// test (a solid type), so the in-place update is race-free.
 template<int Q, uint32_t ST>
 __global__ __launch_bounds__(BLOCK) void k_initialize(LbmArgs a) {
 	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -315,13 +337,68 @@ __global__ __launch_bounds__(BLOCK) void k_initialize(LbmArgs a) {
 	const Site s = make_site(x, blockIdx.y, blockIdx.z, a.Nx, a.Ny, a.Nz);
 	const uint64_t n = s.n();
 	float ux = a.u[n], uy = a.u[a.N + n], uz = a.u[2ull * a.N + n];
-	if((a.flags[n] & TYPE_BO) == TYPE_S) {
+	const uint32_t flagsn = a.flags[n], bo = flagsn & TYPE_BO;
+	bool zero = bo == TYPE_S;
+	if(a.ext & EXT_MOVING_BOUNDARIES) {
+		if(bo == TYPE_S) {
+#pragma unroll
+			for(int i = 1; i < Q; i++) zero = zero && (a.flags[neighbour<Q>(s, i)] & TYPE_BO) == TYPE_S;
+		} else if(bo != TYPE_E) {
+			((uint8_t*)a.flags)[n] = (uint8_t)(next_to_moving_solid<Q>(a, s) ? flagsn | TYPE_MS : flagsn & ~TYPE_MS);
+		}
+	}
+	if(zero) {
 		ux = 0.0f; uy = 0.0f; uz = 0.0f;
 		a.u[n] = 0.0f; a.u[a.N + n] = 0.0f; a.u[2ull * a.N + n] = 0.0f;
 	}
 	float feq[Q];
 	calc_feq<Q>(a.rho[n], a.rho[n] - 1.0f, ux, uy, uz, feq);
 	store_f<Q, ST>((typename Storage<ST>::E*)a.fi, a.N, s, 1u, feq);
+}
+
+// apply_moving_boundaries (src/kernel.cpp:1104-1113): a population that arrives from a
+// solid neighbour (from n - e_i for f_i) gains -6 w_i (c_opposite . u_wall), wall density 1.
+template<int Q> __device__ __forceinline__ void moving_boundaries(const LbmArgs& a, const Site& s, float* fhn) {
+#pragma unroll
+	for(int i = 1; i < Q; i += 2) {
+		const float w6 = -6.0f * weight<Q>(i);
+		uint64_t j = neighbour<Q>(s, i + 1);
+		if((a.flags[j] & TYPE_BO) == TYPE_S) fhn[i] = __builtin_fmaf(w6, cdot<Q>(i + 1, a.u[j], a.u[a.N + j], a.u[2ull * a.N + j]), fhn[i]);
+		j = neighbour<Q>(s, i);
+		if((a.flags[j] & TYPE_BO) == TYPE_S) fhn[i + 1] = __builtin_fmaf(w6, cdot<Q>(i, a.u[j], a.u[a.N + j], a.u[2ull * a.N + j]), fhn[i + 1]);
+	}
+}
+// update_moving_boundaries (src/kernel.cpp:1433-1449): re-mark TYPE_MS after wall velocities
+// changed; not on solid, TYPE_E or TYPE_T cells.
+template<int Q>
+__global__ __launch_bounds__(BLOCK) void k_update_moving_boundaries(LbmArgs a) {
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+	if(x >= a.Nx) return;
+	const Site s = make_site(x, blockIdx.y, blockIdx.z, a.Nx, a.Ny, a.Nz);
+	const uint64_t n = s.n();
+	const uint32_t flagsn = a.flags[n], bo = flagsn & TYPE_BO;
+	if(bo == TYPE_S || bo == TYPE_E || (flagsn & TYPE_T)) return;
+	((uint8_t*)a.flags)[n] = (uint8_t)(next_to_moving_solid<Q>(a, s) ? flagsn | TYPE_MS : flagsn & ~TYPE_MS);
+}
+// SUBGRID (src/kernel.cpp:1579-1593), Smagorinsky-Lilly: local relaxation rate from the
+// non-equilibrium stress H = sum_i c_i c_i (f_i - feq_i), i >= 1 (zero-coefficient terms
+// left out), w = 2/(tau0 + sqrt(tau0^2 + 0.76421222 sqrt(H:H)/rho)), tau0 = 1/w.
+template<int Q> __device__ __forceinline__ float subgrid_rate(float w, const float* fhn, const float* feq, float rhon) {
+	const float tau0 = 1.0f / w;
+	float Hxx = 0.0f, Hyy = 0.0f, Hzz = 0.0f, Hxy = 0.0f, Hxz = 0.0f, Hyz = 0.0f;
+#pragma unroll
+	for(int i = 1; i < Q; i++) {
+		const float fneq = fhn[i] - feq[i];
+		const int ax = VS<Q>::cx[i], ay = VS<Q>::cy[i], az = VS<Q>::cz[i];
+		if(ax != 0) Hxx += fneq;
+		if(ax * ay != 0) Hxy = ax * ay > 0 ? Hxy + fneq : Hxy - fneq;
+		if(ay != 0) Hyy += fneq;
+		if(ax * az != 0) Hxz = ax * az > 0 ? Hxz + fneq : Hxz - fneq;
+		if(ay * az != 0) Hyz = ay * az > 0 ? Hyz + fneq : Hyz - fneq;
+		if(az != 0) Hzz += fneq;
+	}
+	const float Qs = ((Hxx * Hxx + Hyy * Hyy) + Hzz * Hzz) + 2.0f * ((Hxy * Hxy + Hxz * Hxz) + Hyz * Hyz);
+	return 2.0f / (tau0 + sqrtf(tau0 * tau0 + SMAGORINSKY * sqrtf(Qs) / rhon));
 }
 
 // stream_collide (src/kernel.cpp:1454-1636).  EXT is a compile-time mask of the
@@ -367,7 +444,8 @@ __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint3
 	}
 	float feq[Q];
 	calc_feq<Q>(rhon, rhom1, uxn, uyn, uzn, feq);
-	const float w = a.w;
+	float w = a.w;
+	if(a.ext & EXT_SUBGRID) w = subgrid_rate<Q>(w, fhn, feq, rhon);
 	if constexpr(!TRT) { // src/kernel.cpp:1595-1604
 		if(vf) {
 			const float c_tau = __builtin_fmaf(w, -0.5f, 1.0f);
@@ -454,6 +532,7 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 	} else {
 		load_f<Q, ST>(fi, a.N, s, a.odd, f);
 	}
+	if((a.ext & EXT_MOVING_BOUNDARIES) && bo == TYPE_MS) moving_boundaries<Q>(a, s, f);
 	collide_cell<Q, TRT, EXT, RECORD>(a, n, bo, f, moments);
 	store_f<Q, ST>(fi, a.N, s, a.odd, f);
 }
@@ -483,6 +562,7 @@ __global__ __launch_bounds__(BLOCK) void k_update_fields(LbmArgs a) {
 	if(bo == TYPE_S || (flagsn & TYPE_SU) == TYPE_G) return;
 	float fhn[Q];
 	load_f<Q, ST>((const typename Storage<ST>::E*)a.fi, a.N, s, a.odd, fhn);
+	if((a.ext & EXT_MOVING_BOUNDARIES) && bo == TYPE_MS) moving_boundaries<Q>(a, s, fhn);
 	float rhon, uxn, uyn, uzn;
 	calc_rho_u<Q>(fhn, rhon, uxn, uyn, uzn);
 	float fxn = a.fx, fyn = a.fy, fzn = a.fz;

@@ -67,9 +67,11 @@ struct LbmKernels {
 	objective_fn objective_partial; // grid of ceil(Ny*Nz/256) blocks, one partial each
 	bool stream_collide_xcd_map;    // 1-D XCD-mapped grid for rows of several blocks
 	transfer_fn transfer_fi;        // halo transfer of the DDFs (k_transfer_fi)
+	lattice_fn update_moving_boundaries;
 };
 
-constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY | EXT_PRECISE_DENSITY;
+constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY |
+                             EXT_PRECISE_DENSITY | EXT_MOVING_BOUNDARIES | EXT_SUBGRID;
 
 // Measured launch tuning (see KernelSet), MI355X, 256^3 to 1536x768x768:
 // - the plain FP32 kernel (non-temporal DDF access) streams fastest with fewer concurrent
@@ -98,6 +100,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static void update_fields(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_fields<Q, ST, (EXT == 0u ? 0u : EXT_ALL)><<<s.grid, s.block, 0, st>>>(a); }
 	static void update_force_field(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_force_field<Q, ST><<<s.grid, s.block, 0, st>>>(a); }
 	static void mass_partial(const LbmArgs& a, const LaunchShape& s, hipStream_t st, double* p) { k_mass_partial<Q, ST><<<s.grid, s.block, 0, st>>>(a, p); }
+	static void update_moving_boundaries(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_moving_boundaries<Q><<<s.grid, s.block, 0, st>>>(a); }
 	static void stream_collide_record(const LbmArgs& a, const LaunchShape& s, hipStream_t st, float* m) { k_stream_collide_record<Q, ST, TRT><<<s.grid, s.block, 0, st>>>(a, m); }
 	static void adjoint_step(const AdjointArgs& g, const LaunchShape& s, hipStream_t st) { k_adjoint_step<Q, TRT><<<s.grid, s.block, 0, st>>>(g); }
 	static void objective_partial(const LbmArgs& a, uint32_t plane_x, hipStream_t st, double* p) {
@@ -121,7 +124,8 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	}
 	static const LbmKernels* table() {
 		static const LbmKernels k = {initialize, stream_collide, update_fields, update_force_field, mass_partial, waves_cap,
-		                             stream_collide_record, adjoint_step, objective_partial, xcd_map, transfer_fi};
+		                             stream_collide_record, adjoint_step, objective_partial, xcd_map, transfer_fi,
+		                             update_moving_boundaries};
 		return &k;
 	}
 };

@@ -25,9 +25,12 @@
 //#define UPDATE_FIELDS // rho/u written by every stream_collide
 //#define POROSITY // per-cell Brinkman porosity lbm.porosity (not in the reference)
 //#define PRECISE_DENSITY // equilibrium takes rho-1 as the DDF sum (not in the reference; FP32 accuracy at tiny forces)
+//#define MOVING_BOUNDARIES // TYPE_S cells with a velocity move (TYPE_MS marks their neighbours)
+//#define SUBGRID // Smagorinsky-Lilly subgrid turbulence model
 
 #define TYPE_S 0x01 // (stationary) solid boundary
 #define TYPE_E 0x02 // equilibrium boundary (inflow/outflow)
+#define TYPE_MS 0x03 // cell next to a moving solid boundary (MOVING_BOUNDARIES)
 #define TYPE_T 0x04 // temperature boundary (no TEMPERATURE extension in this build)
 #define TYPE_F 0x08 // fluid (SURFACE, not in this build)
 #define TYPE_I 0x10 // interface (SURFACE, not in this build)
@@ -42,6 +45,8 @@
 #undef EQUILIBRIUM_BOUNDARIES
 #undef POROSITY
 #undef PRECISE_DENSITY
+#undef MOVING_BOUNDARIES
+#undef SUBGRID
 #endif
 
 #if defined(FP16S) && defined(FP16C)
@@ -53,8 +58,8 @@
 #if defined(SRT) && defined(TRT)
 #error "choose one collision operator"
 #endif
-#if defined(MOVING_BOUNDARIES) || defined(SURFACE) || defined(TEMPERATURE) || defined(SUBGRID) || defined(PARTICLES) || defined(GRAPHICS) || defined(INTERACTIVE_GRAPHICS) || defined(INTERACTIVE_GRAPHICS_ASCII)
-#error "this build implements the LBM core (VOLUME_FORCE, FORCE_FIELD, EQUILIBRIUM_BOUNDARIES, UPDATE_FIELDS, POROSITY); the requested extension is not available"
+#if defined(SURFACE) || defined(TEMPERATURE) || defined(PARTICLES) || defined(GRAPHICS) || defined(INTERACTIVE_GRAPHICS) || defined(INTERACTIVE_GRAPHICS_ASCII)
+#error "this build implements the LBM core (VOLUME_FORCE, FORCE_FIELD, EQUILIBRIUM_BOUNDARIES, MOVING_BOUNDARIES, UPDATE_FIELDS, SUBGRID, POROSITY); the requested extension is not available"
 #endif
 
 #if defined(FP16S) || defined(FP16C)

@@ -84,6 +84,12 @@ fx_lbm_config lbm_config(const uint Nx, const uint Ny, const uint Nz, const floa
 #ifdef PRECISE_DENSITY
 	c.extensions |= FX_PRECISE_DENSITY;
 #endif
+#ifdef MOVING_BOUNDARIES
+	c.extensions |= FX_MOVING_BOUNDARIES;
+#endif
+#ifdef SUBGRID
+	c.extensions |= FX_SUBGRID;
+#endif
 	c.w = fx_relaxation_rate(nu);
 	return c;
 }
@@ -175,7 +181,11 @@ void LBM::sanity_checks_initialization() {
 		if((f & (TYPE_S | TYPE_E)) == TYPE_S && (u.x[n] != 0.0f || u.y[n] != 0.0f || u.z[n] != 0.0f)) moving_used = true;
 		if(f & (TYPE_T | TYPE_F | TYPE_I | TYPE_G)) unsupported = true;
 	}
-	if(moving_used) print_warning("Some boundary cells have non-zero velocity, but MOVING_BOUNDARIES is not available in this build; they are treated as stationary.");
+#ifndef MOVING_BOUNDARIES
+	if(moving_used) print_warning("Some boundary cells have non-zero velocity, but MOVING_BOUNDARIES is not enabled. If you intend to use moving boundaries, uncomment \"#define MOVING_BOUNDARIES\" in defines.hpp.");
+#else
+	if(!moving_used) print_warning("The MOVING_BOUNDARIES extension is enabled but no moving boundary cells (TYPE_S flag and velocity unequal to zero) are placed in the simulation box.");
+#endif
 #ifndef EQUILIBRIUM_BOUNDARIES
 	if(equilibrium_used) print_error("Some cells are set as equilibrium boundaries with the TYPE_E flag, but EQUILIBRIUM_BOUNDARIES is not enabled. Uncomment \"#define EQUILIBRIUM_BOUNDARIES\" in defines.hpp.");
 #else
@@ -198,13 +208,22 @@ void LBM::initialize() {
 #ifdef POROSITY
 	porosity.write_to_device();
 #endif
-	fx_check(fx_lbm_initialize(ctx, fi, (const float*)rho.device_buffer(), (float*)u.device_buffer(), (const uint8_t*)flags.device_buffer()), "initialize");
+	fx_check(fx_lbm_initialize(ctx, fi, (const float*)rho.device_buffer(), (float*)u.device_buffer(), (uint8_t*)flags.device_buffer()), "initialize");
 	fx_check(fx_sync(ctx), "initialize");
 	t = 0ull;
 	t_last_update_fields = max_ulong;
 	t_last_force_field = max_ulong;
 	initialized = true;
 }
+
+#ifdef MOVING_BOUNDARIES
+// mark/unmark cells next to moving solid cells with TYPE_MS after the host changed wall
+// velocities (u written to the device), src/lbm.cpp:1019-1027
+void LBM::update_moving_boundaries() {
+	fx_check(fx_lbm_update_moving_boundaries(ctx, (const float*)u.device_buffer(), (uint8_t*)flags.device_buffer()), "update_moving_boundaries");
+	fx_check(fx_sync(ctx), "update_moving_boundaries");
+}
+#endif
 
 // LBM::run (src/lbm.cpp:955-975).  Steps are launched back to back in batches on the
 // context stream; the per-step time given to Info is the batch average from HIP events.

@@ -110,6 +110,9 @@ public:
 
 	void run(const ulong steps = max_ulong, const ulong total_steps = max_ulong); // initialize on first call, then steps
 	void update_fields(); // rho/u from the DDFs at the current step
+#ifdef MOVING_BOUNDARIES
+	void update_moving_boundaries(); // mark/unmark cells next to TYPE_S cells with velocity!=0 with TYPE_MS
+#endif
 	void reset();         // next run() re-initializes from the host fields
 #ifdef FORCE_FIELD
 	void update_force_field();

@@ -168,6 +168,14 @@ class LBM:
         self._update_fields_if_stale()
         self.ctx.sync()
 
+    def update_moving_boundaries(self):
+        """LBM::update_moving_boundaries (src/lbm.cpp:1019-1027): after wall velocities
+        changed (u of TYPE_S cells written to the device), re-mark TYPE_MS."""
+        if not self.extensions & _abi.FX_MOVING_BOUNDARIES:
+            raise ValueError("MOVING_BOUNDARIES is not enabled")
+        check(self.ctx.lib.fx_lbm_update_moving_boundaries(self.ctx.h, self.u.device, self.flags.device))
+        self.ctx.sync()
+
     def reset(self):
         self.initialized = False
 

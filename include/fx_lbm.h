@@ -59,10 +59,17 @@ extern "C" {
  * (the reference's arithmetic); the adjoint configurations use it because at body forces
  * of 1e-6 per step that rounding is several percent of the flow's momentum input. */
 #define FX_PRECISE_DENSITY 32u
+/* MOVING_BOUNDARIES (src/kernel.cpp:1104-1113, 1433-1449): solid cells with a velocity
+ * move (Ladd's correction on the populations bounced from them, wall density 1); cells next
+ * to them carry TYPE_MS.  SUBGRID (src/kernel.cpp:1579-1593): Smagorinsky-Lilly eddy
+ * viscosity. */
+#define FX_MOVING_BOUNDARIES 64u
+#define FX_SUBGRID 128u
 
 /* cell type bits (src/defines.hpp:53-60) */
 #define FX_TYPE_S 0x01u
 #define FX_TYPE_E 0x02u
+#define FX_TYPE_MS 0x03u /* next to a moving solid cell (src/lbm.cpp:402) */
 #define FX_TYPE_T 0x04u
 #define FX_TYPE_F 0x08u
 #define FX_TYPE_I 0x10u
@@ -137,11 +144,17 @@ int fx_buf_copy(fx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 
 /* ---- LBM kernels (typed launches; replace Kernel::enqueue_run, src/opencl.hpp:616-692) ---- */
 
-/* initialize (src/kernel.cpp:1358-1430): zero u of TYPE_S cells, store feq(rho,u) of
- * every cell with the odd-step Esoteric-Pull convention (t=1).  Bound as
- * kernel_initialize(fi,rho,u,flags) in src/lbm.cpp:127; the host sequencing
+/* initialize (src/kernel.cpp:1358-1430): zero u of TYPE_S cells (with
+ * FX_MOVING_BOUNDARIES: of solid cells with only solid neighbours, and mark TYPE_MS in
+ * flags), store feq(rho,u) of every cell with the odd-step Esoteric-Pull convention (t=1).
+ * Bound as kernel_initialize(fi,rho,u,flags) in src/lbm.cpp:127; the host sequencing
  * (t=1 then t=0, src/lbm.cpp:881-922) is the caller's. */
-int fx_lbm_initialize(fx_ctx* ctx, void* fi, const float* rho, float* u, const uint8_t* flags);
+int fx_lbm_initialize(fx_ctx* ctx, void* fi, const float* rho, float* u, uint8_t* flags);
+
+/* update_moving_boundaries (src/kernel.cpp:1433-1449): re-mark TYPE_MS after wall
+ * velocities changed (FX_MOVING_BOUNDARIES).  Replaces LBM_Domain::
+ * enqueue_update_moving_boundaries (src/lbm.cpp:241-243). */
+int fx_lbm_update_moving_boundaries(fx_ctx* ctx, const float* u, uint8_t* flags);
 
 /* stream_collide (src/kernel.cpp:1454-1636): one time step t, in place (Esoteric-Pull).
  * F (3N floats) is read only with FX_FORCE_FIELD, alpha (N floats) only with

@@ -35,11 +35,12 @@ namespace {
 
 enum : uint32_t {
 	EXT_VOLUME_FORCE = 1u, EXT_FORCE_FIELD = 2u, EXT_EQUILIBRIUM_BOUNDARIES = 4u,
-	EXT_UPDATE_FIELDS = 8u, EXT_POROSITY = 16u, EXT_PRECISE_DENSITY = 32u
+	EXT_UPDATE_FIELDS = 8u, EXT_POROSITY = 16u, EXT_PRECISE_DENSITY = 32u, EXT_MOVING_BOUNDARIES = 64u,
+	EXT_SUBGRID = 128u
 };
 enum : uint32_t { ST_FP32 = 0u, ST_FP16S = 1u, ST_FP16C = 2u, ST_FP64 = 3u };
 // cell type bits, src/defines.hpp:53-60 and src/lbm.cpp:399-411
-constexpr uint8_t TYPE_S = 0x01, TYPE_E = 0x02, TYPE_BO = 0x03, TYPE_SU = 0x38, TYPE_G = 0x20;
+constexpr uint8_t TYPE_S = 0x01, TYPE_E = 0x02, TYPE_MS = 0x03, TYPE_BO = 0x03, TYPE_T = 0x04, TYPE_SU = 0x38, TYPE_G = 0x20;
 
 inline uint32_t fbits(float x) { uint32_t b; std::memcpy(&b, &x, 4); return b; }
 inline float bitsf(uint32_t b) { float x; std::memcpy(&x, &b, 4); return x; }
@@ -201,10 +202,12 @@ template<typename T> struct Lit;
 template<> struct Lit<float> {
 	static float c() { return 0.57735027f; } // def_c, src/lbm.cpp:366
 	static float third() { return 0.33333334f; } // src/kernel.cpp:1093-1099
+	static float smagorinsky() { return 0.76421222f; } // 18*sqrt(2)*(C*Delta)^2, src/kernel.cpp:1591
 };
 template<> struct Lit<double> {
 	static double c() { return 0.57735027; }
 	static double third() { return 1.0 / 3.0; }
+	static double smagorinsky() { return 0.76421222; }
 };
 
 // Dual number (value, derivative along one direction) for forward-mode derivatives of the
@@ -224,12 +227,16 @@ inline Dual& operator*=(Dual& a, Dual b) { a = a * b; return a; }
 template<> struct Lit<Dual> {
 	static Dual c() { return Dual(0.57735027); }
 	static Dual third() { return Dual(1.0 / 3.0); }
+	static Dual smagorinsky() { return Dual(0.76421222); }
 };
+inline Dual sqrtT(Dual a) { const double r = std::sqrt(a.v); return Dual(r, a.d / (2.0 * r)); }
 inline Dual fmaT(Dual a, Dual b, Dual c) { return Dual(std::fma(a.v, b.v, c.v), a.d * b.v + a.v * b.d + c.d); }
 inline Dual clampT(Dual x, Dual lo, Dual hi) { return x.v < lo.v ? lo : x.v > hi.v ? hi : x; }
 
 inline float fmaT(float a, float b, float c) { return std::fmaf(a, b, c); }
 inline double fmaT(double a, double b, double c) { return std::fma(a, b, c); }
+inline float sqrtT(float a) { return std::sqrt(a); }   // correctly rounded
+inline double sqrtT(double a) { return std::sqrt(a); }
 template<typename T> inline T clampT(T x, T lo, T hi) { return std::fmin(std::fmax(x, lo), hi); } // OpenCL clamp
 
 // c_i.v evaluated like the reference's hand-written projections (u0=ux+uy, u3=ux-uy,
@@ -340,6 +347,27 @@ template<typename T> void force_block(uint32_t ext, T rhon, T& uxn, T& uyn, T& u
 	}
 }
 
+// SUBGRID, src/kernel.cpp:1579-1593: Smagorinsky-Lilly eddy viscosity.  The local
+// relaxation rate follows from the non-equilibrium stress H = sum_i c_i c_i (f_i - feq_i)
+// (i >= 1): w = 2/(tau0 + sqrt(tau0^2 + 0.76421222*sqrt(H:H)/rho)), tau0 = 1/w.  Terms
+// whose lattice coefficient is 0 are left out (they add exact zeros).
+template<int Q, typename T> T subgrid_rate(T w, const T* fhn, const T* feq, T rhon) {
+	const T tau0 = T(1) / w;
+	T Hxx = T(0), Hyy = T(0), Hzz = T(0), Hxy = T(0), Hxz = T(0), Hyz = T(0);
+	for(int i = 1; i < Q; i++) {
+		const T fneq = fhn[i] - feq[i];
+		const int cx = VS<Q>::cx[i], cy = VS<Q>::cy[i], cz = VS<Q>::cz[i];
+		if(cx != 0) Hxx += fneq;
+		if(cx * cy != 0) Hxy = cx * cy > 0 ? Hxy + fneq : Hxy - fneq;
+		if(cy != 0) Hyy += fneq;
+		if(cx * cz != 0) Hxz = cx * cz > 0 ? Hxz + fneq : Hxz - fneq;
+		if(cy * cz != 0) Hyz = cy * cz > 0 ? Hyz + fneq : Hyz - fneq;
+		if(cz != 0) Hzz += fneq;
+	}
+	const T Qs = ((Hxx * Hxx + Hyy * Hyy) + Hzz * Hzz) + T(2) * ((Hxy * Hxy + Hxz * Hxz) + Hyz * Hyz);
+	return T(2) / (tau0 + sqrtT(tau0 * tau0 + Lit<T>::smagorinsky() * sqrtT(Qs) / rhon));
+}
+
 // Collision of one cell, src/kernel.cpp:1474-1633: moments (or the TYPE_E presets), force
 // block, Guo forcing terms, equilibrium, SRT or TRT, in place on the loaded populations
 // fhn.  fxn.. is the external force of the cell (volume force plus force field).  If
@@ -363,6 +391,7 @@ template<int Q, typename T> void collide_cell(const Cfg& c, T* fhn, bool isE, T 
 	else for(int i = 0; i < Q; i++) Fin[i] = T(0);
 	if(fields) { fields[0] = rhon; fields[1] = uxn; fields[2] = uyn; fields[3] = uzn; }
 	calc_feq<Q, T>(rhon, rhom1, uxn, uyn, uzn, feq);
+	if(c.ext & EXT_SUBGRID) w = subgrid_rate<Q, T>(w, fhn, feq, rhon);
 	if(!c.trt) { // SRT, src/kernel.cpp:1595-1604
 		if(vf) { const T c_tau = fmaT(w, T(-0.5), T(1)); for(int i = 0; i < Q; i++) Fin[i] *= c_tau; }
 		#pragma GCC unroll 32
@@ -460,17 +489,76 @@ template<typename T, int Q, int ST> struct Lattice {
 		}
 	}
 
-	// initialize kernel, src/kernel.cpp:1358-1430 (no MOVING_BOUNDARIES / SURFACE / TEMPERATURE)
-	void initialize(E* fi, const T* rho, T* u, const uint8_t* flags) const {
+	// A non-solid cell next to a solid cell with non-zero velocity, src/kernel.cpp:1395-1398
+	// and 1441-1444 (TYPE_MS marking)
+	bool next_to_moving_solid(const uint64_t* j, const T* u, const uint8_t* flags) const {
+		bool m = false;
+		for(int i = 1; i < Q; i++) {
+			const uint64_t k = j[i];
+			m = m || ((flags[k] & TYPE_BO) == TYPE_S && (u[k] != T(0) || u[N + k] != T(0) || u[2 * N + k] != T(0)));
+		}
+		return m;
+	}
+	// initialize kernel, src/kernel.cpp:1358-1430 (no SURFACE / TEMPERATURE).  Without
+	// MOVING_BOUNDARIES every solid cell's velocity is zeroed; with it only solid cells
+	// whose neighbours are all solid, and the other non-TYPE_E cells are marked or unmarked
+	// TYPE_MS.  New flags are formed from the old ones first and written afterwards (the
+	// reference updates in place; the marks never change the solid test it reads).
+	void initialize(E* fi, const T* rho, T* u, uint8_t* flags) const {
+		const bool mb = c.ext & EXT_MOVING_BOUNDARIES;
+		std::vector<uint8_t> nf(flags, flags + N);
+		parallel_cells(N, c.threads, [&](uint64_t a, uint64_t b) {
+			uint64_t j[Q];
+			for(uint64_t n = a; n < b; n++) {
+				const uint8_t bo = flags[n] & TYPE_BO;
+				if(bo == TYPE_S) {
+					if(!mb) { u[n] = T(0); u[N + n] = T(0); u[2 * N + n] = T(0); continue; }
+					neighbors(n, j);
+					bool only_s = true;
+					for(int i = 1; i < Q; i++) only_s = only_s && (flags[j[i]] & TYPE_BO) == TYPE_S;
+					if(only_s) { u[n] = T(0); u[N + n] = T(0); u[2 * N + n] = T(0); }
+				} else if(mb && bo != TYPE_E) {
+					neighbors(n, j);
+					nf[n] = next_to_moving_solid(j, u, flags) ? (uint8_t)(flags[n] | TYPE_MS) : (uint8_t)(flags[n] & ~TYPE_MS);
+				}
+			}
+		});
+		std::memcpy(flags, nf.data(), N);
 		parallel_cells(N, c.threads, [&](uint64_t a, uint64_t b) {
 			uint64_t j[Q]; T feq[Q];
 			for(uint64_t n = a; n < b; n++) {
 				neighbors(n, j);
-				if((flags[n] & TYPE_BO) == TYPE_S) { u[n] = T(0); u[N + n] = T(0); u[2 * N + n] = T(0); }
 				calc_feq<Q, T>(rho[n], rho[n] - T(1), u[n], u[N + n], u[2 * N + n], feq);
 				store_f(n, feq, fi, j, 1u);
 			}
 		});
+	}
+	// update_moving_boundaries kernel, src/kernel.cpp:1433-1449 (not on solid, TYPE_E or
+	// TYPE_T cells)
+	void update_moving_boundaries(const T* u, uint8_t* flags) const {
+		std::vector<uint8_t> nf(flags, flags + N);
+		parallel_cells(N, c.threads, [&](uint64_t a, uint64_t b) {
+			uint64_t j[Q];
+			for(uint64_t n = a; n < b; n++) {
+				const uint8_t bo = flags[n] & TYPE_BO;
+				if(bo == TYPE_S || bo == TYPE_E || (flags[n] & TYPE_T)) continue;
+				neighbors(n, j);
+				nf[n] = next_to_moving_solid(j, u, flags) ? (uint8_t)(flags[n] | TYPE_MS) : (uint8_t)(flags[n] & ~TYPE_MS);
+			}
+		});
+		std::memcpy(flags, nf.data(), N);
+	}
+	// apply_moving_boundaries, src/kernel.cpp:1104-1113: a population that arrives from a
+	// solid neighbour (n - e_i for f_i) gains -6 w_i (c_opposite . u_wall), wall density 1
+	// (Krueger et al. p. 180)
+	void apply_moving_boundaries(T* fhn, const uint64_t* j, const T* u, const uint8_t* flags) const {
+		for(int i = 1; i < Q; i += 2) {
+			const T w6 = T(-6) * weight<Q, T>(i);
+			uint64_t k = j[i + 1];
+			if((flags[k] & TYPE_BO) == TYPE_S) fhn[i] = fmaT(w6, project<Q, T>(i + 1, u[k], u[N + k], u[2 * N + k]), fhn[i]);
+			k = j[i];
+			if((flags[k] & TYPE_BO) == TYPE_S) fhn[i + 1] = fmaT(w6, project<Q, T>(i, u[k], u[N + k], u[2 * N + k]), fhn[i + 1]);
+		}
 	}
 
 	T relaxation_rate() const { return (T)(std::is_same<T, float>::value ? (double)(float)c.w : c.w); }
@@ -487,6 +575,7 @@ template<typename T, int Q, int ST> struct Lattice {
 				if(bo == TYPE_S || su == TYPE_G) continue;
 				neighbors(n, j);
 				load_f(n, fhn, fi, j, t);
+				if((c.ext & EXT_MOVING_BOUNDARIES) && bo == TYPE_MS) apply_moving_boundaries(fhn, j, u, flags);
 				const bool isE = eq && bo == TYPE_E;
 				T fxn = fx, fyn = fy, fzn = fz;
 				if(c.ext & EXT_FORCE_FIELD) { fxn += F[n]; fyn += F[N + n]; fzn += F[2 * N + n]; }
@@ -510,6 +599,7 @@ template<typename T, int Q, int ST> struct Lattice {
 				if(bo == TYPE_S || su == TYPE_G) continue;
 				neighbors(n, j);
 				load_f(n, fhn, fi, j, t);
+				if((c.ext & EXT_MOVING_BOUNDARIES) && bo == TYPE_MS) apply_moving_boundaries(fhn, j, u, flags);
 				T rhon, uxn, uyn, uzn;
 				calc_rho_u<Q, T>(fhn, &rhon, &uxn, &uyn, &uzn);
 				T fxn = fx, fyn = fy, fzn = fz;
@@ -654,8 +744,14 @@ template<template<typename, int, int> class Body, typename... A> int dispatch(co
 }
 
 template<typename T, int Q, int ST> struct DoInit {
-	static int run(const Cfg& c, void* fi, const void* rho, void* u, const uint8_t* flags) {
+	static int run(const Cfg& c, void* fi, const void* rho, void* u, uint8_t* flags) {
 		Lattice<T, Q, ST>(c).initialize((typename Store<T, ST>::E*)fi, (const T*)rho, (T*)u, flags);
+		return 0;
+	}
+};
+template<typename T, int Q, int ST> struct DoMovingBoundaries {
+	static int run(const Cfg& c, const void* u, uint8_t* flags) {
+		Lattice<T, Q, ST>(c).update_moving_boundaries((const T*)u, flags);
 		return 0;
 	}
 };
@@ -751,9 +847,14 @@ void fxo_decode_array(int storage, const uint16_t* h, float* out, uint64_t n) {
 	for(uint64_t k = 0; k < n; k++) out[k] = storage == (int)ST_FP16S ? fp16s_load(h[k]) : storage == (int)ST_FP16C ? fp16c_load(h[k]) : f16_to_f32(h[k]);
 }
 
-int fxo_initialize(const fxo_cfg* c, void* fi, const void* rho, void* u, const uint8_t* flags) {
+// flags is written with MOVING_BOUNDARIES (TYPE_MS marks)
+int fxo_initialize(const fxo_cfg* c, void* fi, const void* rho, void* u, uint8_t* flags) {
 	if(!valid(c)) return -1;
 	return dispatch<DoInit>(*c, fi, rho, u, flags);
+}
+int fxo_update_moving_boundaries(const fxo_cfg* c, const void* u, uint8_t* flags) {
+	if(!valid(c)) return -1;
+	return dispatch<DoMovingBoundaries>(*c, u, flags);
 }
 int fxo_stream_collide(const fxo_cfg* c, void* fi, void* rho, void* u, const uint8_t* flags, uint64_t t, double fx, double fy, double fz, const void* F, const void* alpha) {
 	if(!valid(c)) return -1;
@@ -775,6 +876,7 @@ int fxo_update_force_field(const fxo_cfg* c, const void* fi, const uint8_t* flag
 int fxo_adjoint_step(const fxo_cfg* c, const void* f_state, void* lambda, const uint8_t* flags, uint64_t t, double fx, double fy, double fz,
                      const void* F, const void* alpha, uint32_t plane_x, double obj_weight, void* grad, const void* rho, const void* u) {
 	if(!valid(c) || c->storage != ST_FP64) return -1;
+	if(c->ext & (EXT_MOVING_BOUNDARIES | EXT_SUBGRID)) return -3; // not differentiated
 	if((c->ext & EXT_POROSITY) && (!alpha || !grad)) return -2;
 	if((c->ext & EXT_FORCE_FIELD) && !F) return -2;
 	if((c->ext & EXT_EQUILIBRIUM_BOUNDARIES) && (!rho || !u)) return -2;

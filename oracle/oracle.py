@@ -24,9 +24,12 @@ EXT_EQUILIBRIUM_BOUNDARIES = 4
 EXT_UPDATE_FIELDS = 8
 EXT_POROSITY = 16
 EXT_PRECISE_DENSITY = 32
+EXT_MOVING_BOUNDARIES = 64
+EXT_SUBGRID = 128
 
 TYPE_S = 0x01
 TYPE_E = 0x02
+TYPE_MS = 0x03
 TYPE_X = 0x40
 
 
@@ -60,6 +63,7 @@ def lib():
         L.fxo_encode_array.argtypes = [ctypes.c_int, vp, vp, ctypes.c_uint64]
         L.fxo_decode_array.argtypes = [ctypes.c_int, vp, vp, ctypes.c_uint64]
         L.fxo_initialize.argtypes = [cp, vp, vp, vp, u8p]
+        L.fxo_update_moving_boundaries.argtypes = [cp, vp, u8p]
         step_args = [cp, vp, vp, vp, u8p, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp, vp]
         L.fxo_stream_collide.argtypes = step_args
         L.fxo_update_fields.argtypes = step_args
@@ -145,6 +149,10 @@ class OracleLBM:
         rc = lib().fxo_initialize(ctypes.byref(self.cfg), _ptr(self.fi), _ptr(self.rho), _ptr(self.u), self.flags_ptr)
         assert rc == 0, rc
         self.t = 0
+
+    def update_moving_boundaries(self):
+        """TYPE_MS marks of the cells next to moving solid cells (src/kernel.cpp:1433-1449)."""
+        assert lib().fxo_update_moving_boundaries(ctypes.byref(self.cfg), _ptr(self.u), self.flags_ptr) == 0
 
     def step(self, steps=1):
         for _ in range(steps):

@@ -24,9 +24,13 @@ EXT_SETS = {
     "equilibrium": ("EQUILIBRIUM_BOUNDARIES",),
     "volume_force": ("VOLUME_FORCE",),
     "drag": ("EQUILIBRIUM_BOUNDARIES", "FORCE_FIELD"),
+    # solid cells keep their random velocities: moving walls
+    "moving": ("MOVING_BOUNDARIES", "EQUILIBRIUM_BOUNDARIES", "UPDATE_FIELDS"),
+    "subgrid": ("SUBGRID", "VOLUME_FORCE"),
+    "moving_subgrid": ("MOVING_BOUNDARIES", "SUBGRID", "FORCE_FIELD", "VOLUME_FORCE"),
 }
 EXT_BITS = {"VOLUME_FORCE": 1, "FORCE_FIELD": 2, "EQUILIBRIUM_BOUNDARIES": 4, "UPDATE_FIELDS": 8, "POROSITY": 16,
-            "PRECISE_DENSITY": 32}
+            "PRECISE_DENSITY": 32, "MOVING_BOUNDARIES": 64, "SUBGRID": 128}
 
 
 def ext_bits(names):
@@ -96,6 +100,9 @@ def test_stream_collide_matches_oracle(oracle, q, collision, storage, ext, grid)
         lbm.initialize()
         ref.initialize()
         compare_state(lbm, ref, oracle, storage, "after initialize")
+        if "MOVING_BOUNDARIES" in names:  # TYPE_MS marks written by initialize
+            lbm.flags.read_from_device()
+            assert np.array_equal(lbm.flags.host, ref.flags) and (ref.flags & 3 == 3).any()
         for s in range(7):
             lbm.run(1)
             ref.step(1)
