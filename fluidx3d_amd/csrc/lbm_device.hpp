@@ -84,27 +84,27 @@ template<> struct Storage<ST_FP16S> {
 	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { *elem(p, o) = encode(x); }
 };
 // FP16C: 1 sign, 4 exponent (bias 15), 11 mantissa bits (src/kernel.cpp:848-859).
-// Decode without a leading-zero count: a normal code is the top bits of a binary32 with
-// the exponent rebiased by 112; a denormal code is m*2^-25, exact through the integer
-// to float conversion.  Encode is the reference's device rule (add 0x800, truncate; no
-// saturation), written branch-free.
+// Decode: the code's exponent and mantissa fields placed at the binary32 positions give
+// 2^-112 times the value, for normal codes and (as a binary32 denormal) for denormal
+// codes alike; one exact multiplication by 2^112 finishes it (FP32 denormals are kept).
+// Encode is the reference's device rule (add 0x800, truncate; no saturation), written
+// branch-free: the normal code is the shifted bits minus the exponent bias difference,
+// masked like the reference's 4-bit field; the denormal shift is clamped at 31, which
+// yields the reference's zero for every exponent below its denormal range.  Both match
+// the previous, literal formulation on every input (all 2^16 codes, all 2^32 floats).
 template<> struct Storage<ST_FP16C> {
 	typedef uint16_t E;
 	__device__ static __forceinline__ float decode(uint16_t h) {
-		const uint32_t a = (uint32_t)h & 0x7FFFu;
-		const float normal = __uint_as_float((a << 12) + (112u << 23));
-		const float denormal = (float)a * 2.98023223876953125E-8f; // a*2^-25
-		const float v = a >= 0x800u ? normal : denormal;
-		return __uint_as_float(__float_as_uint(v) | (((uint32_t)h & 0x8000u) << 16));
+		const uint32_t bits = (((uint32_t)h & 0x7FFFu) << 12) | (((uint32_t)h & 0x8000u) << 16);
+		return __uint_as_float(bits) * 0x1p112f;
 	}
 	__device__ static __forceinline__ uint16_t encode(float x) {
 		const uint32_t b = __float_as_uint(x) + 0x00000800u;
 		const uint32_t e = (b >> 23) & 0xFFu;
-		const uint32_t m = b & 0x007FFFFFu;
-		const uint32_t normal = ((((e - 112u) << 11) & 0x7800u) | (m >> 12));
-		const uint32_t shift = 124u - e; // 12..23 where used
-		const uint32_t denormal = (((0x007FF800u + m) >> (shift & 31u)) + 1u) >> 1;
-		const uint32_t mag = e > 112u ? normal : (e > 100u ? denormal : 0u);
+		const uint32_t normal = (((b >> 12) & 0x7FFFFu) - 0x38000u) & 0x7FFFu;
+		const uint32_t shift = min(124u - e, 31u);
+		const uint32_t denormal = (((0x007FF800u + (b & 0x007FFFFFu)) >> shift) + 1u) >> 1;
+		const uint32_t mag = e > 112u ? normal : denormal;
 		return (uint16_t)(((b >> 16) & 0x8000u) | mag);
 	}
 	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return *elem(p, o); }
