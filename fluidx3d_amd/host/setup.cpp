@@ -91,6 +91,34 @@ void main_setup() {
 #endif
 }
 
+#elif defined(SCENE_CAVITY)
+#ifndef CAVITY_L
+#define CAVITY_L 128u
+#endif
+#ifndef CAVITY_STEPS
+#define CAVITY_STEPS 10000u
+#endif
+// Lid-driven cavity, the reference's scene (src/setup.cpp:236-251) without graphics: the
+// lid z = Nz-1 moves with u = 0.1 in y, all six faces solid.  Needs MOVING_BOUNDARIES.
+void main_setup() {
+	const uint L = CAVITY_L;
+	const float Re = 1000.0f;
+	const float u = 0.1f;
+	LBM lbm(L, L, L, units.nu_from_Re(Re, (float)(L-2u), u));
+	const uint Nx=lbm.get_Nx(), Ny=lbm.get_Ny(), Nz=lbm.get_Nz(); parallel_for(lbm.get_N(), [&](ulong n) { uint x=0u, y=0u, z=0u; lbm.coordinates(n, x, y, z);
+		if(z==Nz-1) lbm.u.y[n] = u;
+		if(x==0u||x==Nx-1u||y==0u||y==Ny-1u||z==0u||z==Nz-1u) lbm.flags[n] = TYPE_S; // all non periodic
+	});
+	lbm.run(CAVITY_STEPS);
+	lbm.u.read_from_device();
+	char line[256];
+	std::snprintf(line, sizeof(line), "t=%llu centre u=(%.9e, %.9e, %.9e)", (unsigned long long)lbm.get_t(), (double)lbm.u.x[lbm.index(Nx/2u, Ny/2u, Nz/2u)], (double)lbm.u.y[lbm.index(Nx/2u, Ny/2u, Nz/2u)], (double)lbm.u.z[lbm.index(Nx/2u, Ny/2u, Nz/2u)]);
+	println(line);
+#ifdef FX_VTK
+	lbm.u.write_host_to_vtk("cavity_u.vtk", false);
+#endif
+}
+
 #elif defined(SCENE_POROUS_CHANNEL)
 #ifndef CHANNEL_SCALE
 #define CHANNEL_SCALE 1u
@@ -123,6 +151,6 @@ void main_setup() {
 
 #else
 void main_setup() {
-	print_error("no scene selected: compile with -DBENCHMARK, -DSCENE_TAYLOR_GREEN, -DSCENE_SPHERE or -DSCENE_POROUS_CHANNEL");
+	print_error("no scene selected: compile with -DBENCHMARK, -DSCENE_TAYLOR_GREEN, -DSCENE_SPHERE, -DSCENE_CAVITY or -DSCENE_POROUS_CHANNEL");
 }
 #endif
