@@ -655,6 +655,8 @@ int fx_lbm_stream_collide_record(fx_ctx* ctx, void* fi, float* rho, float* u, co
 	const int rc = check_step_args(ctx, fi, rho, u, flags, F, alpha);
 	if(rc != FX_OK) return rc;
 	if(!moments) return fail(FX_ERR_ARG, "null moments");
+	if(ctx->cfg.extensions & (FX_MOVING_BOUNDARIES | FX_SUBGRID))
+		return fail(FX_ERR_CONFIG, "the adjoint does not differentiate MOVING_BOUNDARIES or SUBGRID");
 	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
 	ctx->k->stream_collide_record(make_args(ctx, fi, rho, u, flags, t, fx_, fy_, fz_, F, alpha), ctx->shape, ctx->stream, moments);
 	return check_launch("stream_collide_record");
@@ -676,6 +678,8 @@ int fx_lbm_adjoint_step(fx_ctx* ctx, float* lambda, const float* rho, const floa
 	int rc = check_step_args(ctx, lambda, rho, u, flags, F, alpha);
 	if(rc != FX_OK) return rc;
 	if(!moments) return fail(FX_ERR_ARG, "null moments");
+	if(ctx->cfg.extensions & (FX_MOVING_BOUNDARIES | FX_SUBGRID))
+		return fail(FX_ERR_CONFIG, "the adjoint does not differentiate MOVING_BOUNDARIES or SUBGRID");
 	if((ctx->cfg.extensions & FX_POROSITY) && !grad_alpha) return fail(FX_ERR_ARG, "POROSITY needs grad_alpha");
 	if((ctx->cfg.extensions & FX_EQUILIBRIUM_BOUNDARIES) && (!rho || !u)) return fail(FX_ERR_ARG, "TYPE_E presets rho/u required");
 	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;

@@ -558,9 +558,11 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
 }
 // stream_collide that also writes the pre-collision (rho, jx, jy, jz) of every fluid cell to
 // moments[c*N+n], c = 0..3 (16 B/cell): what the adjoint step needs of the forward state.
-template<int Q, uint32_t ST, bool TRT>
+// EXT is the context's extension subset; MOVING_BOUNDARIES and SUBGRID are not
+// differentiated and compiled out.
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide_record(LbmArgs a, float* moments) {
-	stream_collide_cell<Q, ST, TRT, EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY | EXT_PRECISE_DENSITY, true, false>(a, moments);
+	stream_collide_cell<Q, ST, TRT, EXT & ~(EXT_MOVING_BOUNDARIES | EXT_SUBGRID), true, false>(a, moments);
 }
 
 // update_fields (src/kernel.cpp:1794-1870)

@@ -101,7 +101,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static void update_force_field(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_force_field<Q, ST><<<s.grid, s.block, 0, st>>>(a); }
 	static void mass_partial(const LbmArgs& a, const LaunchShape& s, hipStream_t st, double* p) { k_mass_partial<Q, ST><<<s.grid, s.block, 0, st>>>(a, p); }
 	static void update_moving_boundaries(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_moving_boundaries<Q><<<s.grid, s.block, 0, st>>>(a); }
-	static void stream_collide_record(const LbmArgs& a, const LaunchShape& s, hipStream_t st, float* m) { k_stream_collide_record<Q, ST, TRT><<<s.grid, s.block, 0, st>>>(a, m); }
+	static void stream_collide_record(const LbmArgs& a, const LaunchShape& s, hipStream_t st, float* m) { k_stream_collide_record<Q, ST, TRT, EXT><<<s.grid, s.block, 0, st>>>(a, m); }
 	static void adjoint_step(const AdjointArgs& g, const LaunchShape& s, hipStream_t st) { k_adjoint_step<Q, TRT><<<s.grid, s.block, 0, st>>>(g); }
 	static void objective_partial(const LbmArgs& a, uint32_t plane_x, hipStream_t st, double* p) {
 		const uint64_t cells = (uint64_t)a.Ny * a.Nz;
