@@ -36,6 +36,9 @@ def test_byte_accounting_matches_reference():
     assert bw(19, "FP32", ("UPDATE_FIELDS",)) == 169 and bw(19, "FP32", ("FORCE_FIELD",)) == 165
     assert mem(19, "FP32") == 93 and mem(19, "FP16S") == 55 and mem(27, "FP32") == 125 and mem(27, "FP16C") == 71
     assert mem(19, "FP32", ("FORCE_FIELD",)) == 105
+    # neighbour flags with MOVING_BOUNDARIES (src/lbm.cpp:62-64); SUBGRID moves no bytes
+    assert bw(19, "FP32", ("MOVING_BOUNDARIES",)) == 153 + 18 and bw(27, "FP16S", ("MOVING_BOUNDARIES",)) == 109 + 26
+    assert bw(19, "FP32", ("SUBGRID",)) == 153
 
 
 def test_config_errors_are_status_codes():
@@ -57,6 +60,12 @@ def test_config_errors_are_status_codes():
         assert not h.value
     assert L.fx_ctx_create(0, None, ctypes.byref(h)) == -1
     assert L.fx_lbm_stream_collide(None, None, None, None, None, 0, 0.0, 0.0, 0.0, None, None) == -1
+    # null contexts of the other entry points are argument errors, not crashes
+    assert L.fx_lbm_update_moving_boundaries(None, None, None) == -1
+    assert L.fx_lbm_transfer_extract(None, 0, 0, 0, None, None, None, None, None, None, None) == -1
+    assert L.fx_lbm_transfer_insert(None, 0, 0, 0, None, None, None, None, None, None, None) == -1
+    assert L.fx_transfer_bytes(None, 0, 0) == 0
+    assert L.fx_transfer_bytes(ctypes.byref(_abi.make_config(8, 8, 8, 1.0)), 0, 3) == 0  # no such direction
 
 
 def test_python_host_checks_like_reference():
