@@ -104,6 +104,7 @@ int check_step_args(const fx_ctx* ctx, const void* fi, const float* rho, const f
 	if((ext & FX_FORCE_FIELD) && !F) return fail(FX_ERR_ARG, "FORCE_FIELD is enabled but F is null");
 	if((ext & FX_POROSITY) && !alpha) return fail(FX_ERR_ARG, "POROSITY is enabled but alpha is null");
 	if((ext & (FX_EQUILIBRIUM_BOUNDARIES | FX_UPDATE_FIELDS)) && (!rho || !u)) return fail(FX_ERR_ARG, "rho and u are required with EQUILIBRIUM_BOUNDARIES or UPDATE_FIELDS");
+	if((ext & FX_MOVING_BOUNDARIES) && !u) return fail(FX_ERR_ARG, "u (the wall velocities) is required with MOVING_BOUNDARIES");
 	return FX_OK;
 }
 

@@ -159,7 +159,7 @@ int fx_lbm_update_moving_boundaries(fx_ctx* ctx, const float* u, uint8_t* flags)
 /* stream_collide (src/kernel.cpp:1454-1636): one time step t, in place (Esoteric-Pull).
  * F (3N floats) is read only with FX_FORCE_FIELD, alpha (N floats) only with
  * FX_POROSITY, rho/u only for TYPE_E cells with FX_EQUILIBRIUM_BOUNDARIES and written
- * with FX_UPDATE_FIELDS.  Replaces LBM_Domain::enqueue_stream_collide (src/lbm.cpp:181-183). */
+ * with FX_UPDATE_FIELDS, u of solid cells (wall velocities) with FX_MOVING_BOUNDARIES.  Replaces LBM_Domain::enqueue_stream_collide (src/lbm.cpp:181-183). */
 int fx_lbm_stream_collide(fx_ctx* ctx, void* fi, float* rho, float* u, const uint8_t* flags,
                           uint64_t t, float fx, float fy, float fz, const float* F, const float* alpha);
 
