@@ -40,7 +40,7 @@ EXPORTS = [
     "fx_lbm_mass_sum", "fx_codec_roundtrip", "fx_bench_copy", "fx_adjoint_memory", "fx_adjoint_create",
     "fx_adjoint_destroy", "fx_lbm_forward", "fx_lbm_backward", "fx_lbm_stream_collide_record", "fx_lbm_adjoint_step",
     "fx_lbm_objective_plane", "fx_transfer_bytes", "fx_lbm_transfer_extract", "fx_lbm_transfer_insert",
-    "fx_lbm_update_moving_boundaries",
+    "fx_lbm_update_moving_boundaries", "fx_buf_read_box", "fx_buf_write_box",
 ]
 
 
@@ -98,6 +98,9 @@ def load():
     L.fx_buf_read.argtypes = [vp, vp, u64, u64, vp, i32]
     L.fx_buf_fill.argtypes = [vp, vp, u64, ctypes.c_uint8]
     L.fx_buf_copy.argtypes = [vp, vp, vp, u64]
+    box = [vp, vp, vp, u32, u64, u32, u64, u64, u64, u64, u64, u64, u64, u64, i32, i32]
+    L.fx_buf_read_box.argtypes = box
+    L.fx_buf_write_box.argtypes = box
     L.fx_lbm_initialize.argtypes = [vp, vp, vp, vp, vp]
     L.fx_lbm_update_moving_boundaries.argtypes = [vp, vp, vp]
     step = [vp, vp, vp, vp, vp, u64, f32, f32, f32, vp, vp]
@@ -206,6 +209,18 @@ class Context:
 
     def read(self, src, host_array, offset=0, blocking=True):
         check(self.lib.fx_buf_read(self.h, src, offset, host_array.nbytes, host_array.ctypes.data, 1 if blocking else 0))
+
+    def read_box(self, src, host_array, dims, Nx, Ny, x0, x1, y0=0, y1=1, z0=0, z1=1, dimension=-1, blocking=True):
+        """Box [x0,x1) x [y0,y1) x [z0,z1) of component `dimension` (all if < 0) of a field of
+        `dims` components into the same offsets of host_array (fx_buf_read_box)."""
+        N = host_array.size // dims
+        check(self.lib.fx_buf_read_box(self.h, src, host_array.ctypes.data, host_array.itemsize, N, dims, Nx, Ny, x0, x1,
+                                       y0, y1, z0, z1, dimension, 1 if blocking else 0))
+
+    def write_box(self, dst, host_array, dims, Nx, Ny, x0, x1, y0=0, y1=1, z0=0, z1=1, dimension=-1, blocking=True):
+        N = host_array.size // dims
+        check(self.lib.fx_buf_write_box(self.h, dst, host_array.ctypes.data, host_array.itemsize, N, dims, Nx, Ny, x0, x1,
+                                        y0, y1, z0, z1, dimension, 1 if blocking else 0))
 
     def fill(self, dst, nbytes, value=0):
         check(self.lib.fx_buf_fill(self.h, dst, int(nbytes), value))

@@ -284,6 +284,50 @@ __global__ void k_transfer_field(uint32_t field, char* buffer_p, char* buffer_m,
 	}
 }
 
+// Box copy between a device buffer and a host array of the same layout (elements e at byte
+// e*eb on both sides).  For every component i and slice z, the rows y0..y1-1 start at
+// element i*N + x0 + (y + z*Ny)*Nx and hold x1-x0 elements; rows that end past N*d are cut
+// at N*d and rows that start there are skipped.  The rows of a slice that lie whole below
+// N*d go in one pitched copy (pitch Nx elements) when a row is no wider than the pitch.
+static int box_copy(fx_ctx* ctx, char* device, char* host, bool to_device, uint32_t eb, uint64_t N, uint32_t d, uint64_t Nx,
+                    uint64_t Ny, uint64_t x0, uint64_t x1, uint64_t y0, uint64_t y1, uint64_t z0, uint64_t z1, int component,
+                    int blocking) {
+	if(!ctx || !device || !host) return fail(FX_ERR_ARG, "null argument");
+	if(eb == 0u || N == 0ull || d == 0u || Nx == 0ull || Ny == 0ull) return fail(FX_ERR_ARG, "elem_bytes, N, d, Nx and Ny must be positive");
+	if(x1 < x0 || y1 < y0 || z1 < z0) return fail(FX_ERR_ARG, "box ranges must satisfy x0 <= x1, y0 <= y1, z0 <= z1");
+	if(component >= (int)d) return fail(FX_ERR_ARG, "component must be below the number of components d (or negative for all)");
+	const uint64_t range = N * d, w = x1 - x0;
+	const hipMemcpyKind kind = to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+	auto copy = [&](uint64_t e, uint64_t width, uint64_t pitch, uint64_t rows) -> hipError_t {
+		char* dv = device + e * eb;
+		char* hs = host + e * eb;
+		if(rows == 1ull) return hipMemcpyAsync(to_device ? dv : hs, to_device ? hs : dv, width * eb, kind, ctx->stream);
+		return hipMemcpy2DAsync(to_device ? dv : hs, pitch * eb, to_device ? hs : dv, pitch * eb, width * eb, rows, kind, ctx->stream);
+	};
+	const uint32_t i0 = component < 0 ? 0u : (uint32_t)component, i1 = component < 0 ? d : i0 + 1u;
+	if(w > 0ull && y1 > y0) {
+		for(uint32_t i = i0; i < i1; i++) {
+			for(uint64_t z = z0; z < z1; z++) {
+				const uint64_t first = (uint64_t)i * N + x0 + (y0 + z * Ny) * Nx; // element of row y0
+				if(first >= range) break; // later slices start further on
+				uint64_t whole = 0ull;      // rows from y0 on that end at or below N*d
+				if(first + w <= range) whole = std::min<uint64_t>(y1 - y0, (range - w - first) / Nx + 1ull);
+				if(whole > 0ull) {
+					if(w <= Nx) FX_HIP(copy(first, w, Nx, whole));
+					else for(uint64_t r = 0ull; r < whole; r++) FX_HIP(copy(first + r * Nx, w, Nx, 1ull));
+				}
+				if(whole < y1 - y0) { // the row that N*d cuts, if it starts below N*d
+					const uint64_t e = first + whole * Nx;
+					if(e < range) FX_HIP(copy(e, range - e, Nx, 1ull));
+					break;
+				}
+			}
+		}
+	}
+	if(blocking) FX_HIP(hipStreamSynchronize(ctx->stream));
+	return FX_OK;
+}
+
 extern "C" {
 
 const char* fx_last_error(void) { return g_last_error.c_str(); }
@@ -451,6 +495,16 @@ int fx_buf_read(fx_ctx* ctx, const void* src, uint64_t offset, uint64_t bytes, v
 	FX_HIP(hipMemcpyAsync(host, (const char*)src + offset, bytes, hipMemcpyDeviceToHost, ctx->stream));
 	if(blocking) FX_HIP(hipStreamSynchronize(ctx->stream));
 	return FX_OK;
+}
+
+int fx_buf_read_box(fx_ctx* ctx, const void* src, void* host, uint32_t elem_bytes, uint64_t N, uint32_t d, uint64_t Nx, uint64_t Ny,
+                    uint64_t x0, uint64_t x1, uint64_t y0, uint64_t y1, uint64_t z0, uint64_t z1, int component, int blocking) {
+	return box_copy(ctx, (char*)src, (char*)host, false, elem_bytes, N, d, Nx, Ny, x0, x1, y0, y1, z0, z1, component, blocking);
+}
+
+int fx_buf_write_box(fx_ctx* ctx, void* dst, const void* host, uint32_t elem_bytes, uint64_t N, uint32_t d, uint64_t Nx, uint64_t Ny,
+                     uint64_t x0, uint64_t x1, uint64_t y0, uint64_t y1, uint64_t z0, uint64_t z1, int component, int blocking) {
+	return box_copy(ctx, (char*)dst, (char*)host, true, elem_bytes, N, d, Nx, Ny, x0, x1, y0, y1, z0, z1, component, blocking);
 }
 
 int fx_buf_fill(fx_ctx* ctx, void* dst, uint64_t bytes, uint8_t value) {

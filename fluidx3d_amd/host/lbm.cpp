@@ -350,3 +350,7 @@ void LBM::write_status(const string& path) {
 	std::fwrite(s.data(), 1, s.size(), f);
 	std::fclose(f);
 }
+
+// every member of the two field types, including those no scene calls
+template class Memory_Container<float>;
+template class Memory_Container<uchar>;

@@ -29,6 +29,8 @@ template<typename T> class Memory_Container {
 	vector<T> host;
 	void* device = nullptr;
 	void write_vtk(const string& path, const bool convert_to_si_units);
+	void refresh(); // update_fields when rho/u on the device are stale
+	void box(const bool to_device, const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong z0, const ulong z1, const ulong Nx, const ulong Ny, const int dimension, const bool blocking);
 public:
 	class Pointer {
 		Memory_Container* m = nullptr;
@@ -60,6 +62,15 @@ public:
 	void reset(const T value = (T)0) { std::fill(host.begin(), host.end(), value); }
 	void read_from_device();  // refreshes rho/u on the device first when stale (src/lbm.hpp:390-396)
 	void write_to_device();
+	// box transfers to/from the same offsets of the host array, one component (dimension >= 0)
+	// or all; Memory<T>::read_from_device_1d/2d/3d, write_to_device_1d/2d/3d
+	// (src/opencl.hpp:532-607)
+	void read_from_device_1d(const ulong x0, const ulong x1, const int dimension = -1, const bool blocking = true);
+	void write_to_device_1d(const ulong x0, const ulong x1, const int dimension = -1, const bool blocking = true);
+	void read_from_device_2d(const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong Nx, const ulong Ny, const int dimension = -1, const bool blocking = true);
+	void write_to_device_2d(const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong Nx, const ulong Ny, const int dimension = -1, const bool blocking = true);
+	void read_from_device_3d(const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong z0, const ulong z1, const ulong Nx, const ulong Ny, const ulong Nz, const int dimension = -1, const bool blocking = true);
+	void write_to_device_3d(const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong z0, const ulong z1, const ulong Nx, const ulong Ny, const ulong Nz, const int dimension = -1, const bool blocking = true);
 	void write_host_to_vtk(const string& path = "", const bool convert_to_si_units = true);
 	void write_device_to_vtk(const string& path = "", const bool convert_to_si_units = true);
 };
@@ -189,6 +200,10 @@ template<typename T> void Memory_Container<T>::release() {
 	device = nullptr;
 }
 template<typename T> void Memory_Container<T>::read_from_device() {
+	refresh();
+	fx_check(fx_buf_read(lbm->ctx, device, 0ull, capacity(), host.data(), 1), "reading " + name);
+}
+template<typename T> void Memory_Container<T>::refresh() {
 #ifndef UPDATE_FIELDS
 	if(lbm->initialized && lbm->t_last_update_fields != lbm->t) {
 		fx_check(fx_lbm_update_fields(lbm->ctx, lbm->fi, (float*)lbm->rho.device_buffer(), (float*)lbm->u.device_buffer(), (const uint8_t*)lbm->flags.device_buffer(), lbm->t, lbm->fx, lbm->fy, lbm->fz,
@@ -206,8 +221,20 @@ template<typename T> void Memory_Container<T>::read_from_device() {
 		lbm->t_last_update_fields = lbm->t;
 	}
 #endif
-	fx_check(fx_buf_read(lbm->ctx, device, 0ull, capacity(), host.data(), 1), "reading " + name);
 }
+template<typename T> void Memory_Container<T>::box(const bool to_device, const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong z0, const ulong z1, const ulong Nx, const ulong Ny, const int dimension, const bool blocking) {
+	if(to_device) fx_check(fx_buf_write_box(lbm->ctx, device, host.data(), (uint)sizeof(T), N, d, Nx, Ny, x0, x1, y0, y1, z0, z1, dimension, blocking ? 1 : 0), "writing " + name);
+	else {
+		refresh();
+		fx_check(fx_buf_read_box(lbm->ctx, device, host.data(), (uint)sizeof(T), N, d, Nx, Ny, x0, x1, y0, y1, z0, z1, dimension, blocking ? 1 : 0), "reading " + name);
+	}
+}
+template<typename T> void Memory_Container<T>::read_from_device_1d(const ulong x0, const ulong x1, const int dimension, const bool blocking) { box(false, x0, x1, 0ull, 1ull, 0ull, 1ull, N, 1ull, dimension, blocking); }
+template<typename T> void Memory_Container<T>::write_to_device_1d(const ulong x0, const ulong x1, const int dimension, const bool blocking) { box(true, x0, x1, 0ull, 1ull, 0ull, 1ull, N, 1ull, dimension, blocking); }
+template<typename T> void Memory_Container<T>::read_from_device_2d(const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong Nx, const ulong Ny, const int dimension, const bool blocking) { box(false, x0, x1, y0, y1, 0ull, 1ull, Nx, Ny, dimension, blocking); }
+template<typename T> void Memory_Container<T>::write_to_device_2d(const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong Nx, const ulong Ny, const int dimension, const bool blocking) { box(true, x0, x1, y0, y1, 0ull, 1ull, Nx, Ny, dimension, blocking); }
+template<typename T> void Memory_Container<T>::read_from_device_3d(const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong z0, const ulong z1, const ulong Nx, const ulong Ny, const ulong Nz, const int dimension, const bool blocking) { (void)Nz; box(false, x0, x1, y0, y1, z0, z1, Nx, Ny, dimension, blocking); }
+template<typename T> void Memory_Container<T>::write_to_device_3d(const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong z0, const ulong z1, const ulong Nx, const ulong Ny, const ulong Nz, const int dimension, const bool blocking) { (void)Nz; box(true, x0, x1, y0, y1, z0, z1, Nx, Ny, dimension, blocking); }
 template<typename T> void Memory_Container<T>::write_to_device() {
 	fx_check(fx_buf_write(lbm->ctx, device, 0ull, capacity(), host.data(), 1), "writing " + name);
 }

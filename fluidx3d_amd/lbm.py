@@ -58,9 +58,38 @@ class Field:
 
     def read_from_device(self):
         """update_fields first if rho/u are stale (src/lbm.hpp:390-396), then copy back."""
+        self._refresh()
+        self.lbm.ctx.read(self.device, self.host)
+
+    def _refresh(self):
         if self.name in ("rho", "u") and self.lbm.initialized:
             self.lbm._update_fields_if_stale()
-        self.lbm.ctx.read(self.device, self.host)
+
+    # Box transfers into/out of the same offsets of the host array, one component
+    # (dimension >= 0) or all; Memory<T>::read_from_device_1d/2d/3d and
+    # write_to_device_1d/2d/3d (src/opencl.hpp:532-607).
+    def read_from_device_1d(self, x0, x1, dimension=-1, blocking=True):
+        self._refresh()
+        self.lbm.ctx.read_box(self.device, self.host, self.dims, self.N, 1, x0, x1, dimension=dimension, blocking=blocking)
+
+    def write_to_device_1d(self, x0, x1, dimension=-1, blocking=True):
+        self.lbm.ctx.write_box(self.device, self.host, self.dims, self.N, 1, x0, x1, dimension=dimension, blocking=blocking)
+
+    def read_from_device_2d(self, x0, x1, y0, y1, Nx, Ny, dimension=-1, blocking=True):
+        self._refresh()
+        self.lbm.ctx.read_box(self.device, self.host, self.dims, Nx, Ny, x0, x1, y0, y1, dimension=dimension,
+                              blocking=blocking)
+
+    def write_to_device_2d(self, x0, x1, y0, y1, Nx, Ny, dimension=-1, blocking=True):
+        self.lbm.ctx.write_box(self.device, self.host, self.dims, Nx, Ny, x0, x1, y0, y1, dimension=dimension,
+                               blocking=blocking)
+
+    def read_from_device_3d(self, x0, x1, y0, y1, z0, z1, Nx, Ny, Nz, dimension=-1, blocking=True):
+        self._refresh()
+        self.lbm.ctx.read_box(self.device, self.host, self.dims, Nx, Ny, x0, x1, y0, y1, z0, z1, dimension, blocking)
+
+    def write_to_device_3d(self, x0, x1, y0, y1, z0, z1, Nx, Ny, Nz, dimension=-1, blocking=True):
+        self.lbm.ctx.write_box(self.device, self.host, self.dims, Nx, Ny, x0, x1, y0, y1, z0, z1, dimension, blocking)
 
 
 class LBM:

@@ -142,6 +142,21 @@ int fx_buf_read(fx_ctx* ctx, const void* src, uint64_t offset, uint64_t bytes, v
 int fx_buf_fill(fx_ctx* ctx, void* dst, uint64_t bytes, uint8_t value);
 int fx_buf_copy(fx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 
+/* Box transfers of a field of d components with N elements of elem_bytes each (component c
+ * starts at element c*N; cell n = x + (y + z*Ny)*Nx).  The rows x0..x1-1 of the box
+ * [x0,x1) x [y0,y1) x [z0,z1) of one component (every component when component < 0) move
+ * between the device buffer and the SAME element offsets of the host array, which has the
+ * full N*d layout.  A row that runs past element N*d is cut there, as in
+ * Memory<T>::read_from_device_1d/2d/3d and write_to_device_1d/2d/3d
+ * (src/opencl.hpp:532-607); the 1-D form is Nx = N, Ny = 1, y0 = z0 = 0, y1 = z1 = 1,
+ * the 2-D form z0 = 0, z1 = 1.  Inverted ranges and component >= d are argument errors. */
+int fx_buf_read_box(fx_ctx* ctx, const void* src, void* host, uint32_t elem_bytes, uint64_t N, uint32_t d,
+                    uint64_t Nx, uint64_t Ny, uint64_t x0, uint64_t x1, uint64_t y0, uint64_t y1, uint64_t z0,
+                    uint64_t z1, int component, int blocking);
+int fx_buf_write_box(fx_ctx* ctx, void* dst, const void* host, uint32_t elem_bytes, uint64_t N, uint32_t d,
+                     uint64_t Nx, uint64_t Ny, uint64_t x0, uint64_t x1, uint64_t y0, uint64_t y1, uint64_t z0,
+                     uint64_t z1, int component, int blocking);
+
 /* ---- LBM kernels (typed launches; replace Kernel::enqueue_run, src/opencl.hpp:616-692) ---- */
 
 /* initialize (src/kernel.cpp:1358-1430): zero u of TYPE_S cells (with
