@@ -104,16 +104,23 @@ template<> struct Storage<ST_FP16S> {
 template<> struct Storage<ST_FP16C> {
 	typedef uint16_t E;
 	__device__ static __forceinline__ float decode(uint16_t h) {
-		const uint32_t bits = (((uint32_t)h & 0x7FFFu) << 12) | (((uint32_t)h & 0x8000u) << 16);
+		// sign-extended code << 12 puts the magnitude at bits 12..26 and the sign at 27..31
+		const uint32_t bits = ((uint32_t)(int32_t)(int16_t)h << 12) & 0x87FFF000u;
 		return __uint_as_float(bits) * 0x1p112f;
 	}
+	// With b = bits(x) + 0x800: the normal code's exponent field is (e_b - 112) mod 16 =
+	// e_b mod 16, so the normal magnitude is bits 12..26 of b.  The reference's denormal
+	// code ((0x7FF800 + m_b) >> (124 - e_b) + 1) >> 1 equals floor(|x|*2^25 + 1/2) for every
+	// |x| below 2^-14*(1 - 2^-12) (e_b <= 112; zero from e_b <= 100 on), with or without a
+	// carry out of the mantissa in b, and is taken as (floor(|x|*2^26) + 1) >> 1: the
+	// product is exact and the conversion truncates, so no rounding enters.  A NaN fails the
+	// comparison and takes the normal path, as in the reference.
 	__device__ static __forceinline__ uint16_t encode(float x) {
 		const uint32_t b = __float_as_uint(x) + 0x00000800u;
-		const uint32_t e = (b >> 23) & 0xFFu;
-		const uint32_t normal = (((b >> 12) & 0x7FFFFu) - 0x38000u) & 0x7FFFu;
-		const uint32_t shift = min(124u - e, 31u);
-		const uint32_t denormal = (((0x007FF800u + (b & 0x007FFFFFu)) >> shift) + 1u) >> 1;
-		const uint32_t mag = e > 112u ? normal : denormal;
+		const uint32_t normal = (b >> 12) & 0x7FFFu;
+		const float ax = fabsf(x);
+		const uint32_t denormal = (__float2uint_rz(ax * 0x1p26f) + 1u) >> 1;
+		const uint32_t mag = ax < __uint_as_float(0x387FF800u) ? denormal : normal;
 		return (uint16_t)(((b >> 16) & 0x8000u) | mag);
 	}
 	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return *elem(p, o); }

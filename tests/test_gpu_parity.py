@@ -202,7 +202,8 @@ def test_lattice_beyond_4_gib_matches_its_tile(storage, reps):
 @pytest.mark.parametrize("storage", ["FP16S", "FP16C"])
 def test_codec_matches_oracle(oracle, storage):
     """Device encode/decode against the oracle: every 16-bit code that encodes a finite
-    value round-trips, and 2M random floats across all magnitudes encode identically."""
+    value round-trips, and 2M random floats across all magnitudes encode identically
+    (FP16C: also every float of the four binades where the encoding rule changes form)."""
     import fluidx3d_amd as fx
     from fluidx3d_amd import _abi
     cfg = fx.make_config(4, 4, 4, 1.0)
@@ -214,7 +215,15 @@ def test_codec_matches_oracle(oracle, storage):
         rng = np.random.default_rng(3)
         rnd = rng.integers(0, 2 ** 32, 2 ** 21, dtype=np.uint64).astype(np.uint32).view(np.float32)
         rnd = rnd[np.isfinite(rnd) & (np.abs(rnd) < (2.0 if storage == "FP16C" else 1.99))]
-        x = np.concatenate([vals[finite], rnd, (rng.uniform(-1, 1, 2 ** 20) * 10.0 ** rng.uniform(-12, 0, 2 ** 20)).astype(np.float32)])
+        parts = [vals[finite], rnd, (rng.uniform(-1, 1, 2 ** 20) * 10.0 ** rng.uniform(-12, 0, 2 ** 20)).astype(np.float32)]
+        if storage == "FP16C":
+            # every float of the binades where the reference's rule changes form: zero below
+            # exponent 101, the denormal codes up to 112, the normal codes from 113 on
+            m = np.arange(2 ** 23, dtype=np.uint32)
+            for e in (100, 101, 112, 113):
+                parts.append(((np.uint32(e) << np.uint32(23)) | m).view(np.float32))
+                parts.append(((np.uint32(256 + e) << np.uint32(23)) | m).view(np.float32))
+        x = np.concatenate(parts)
         n = x.size
         dx = ctx.alloc(4 * n); dc = ctx.alloc(2 * n); dd = ctx.alloc(4 * n)
         ctx.write(dx, x)
