@@ -37,14 +37,18 @@ inline LaunchShape stream_shape(uint32_t Nx, uint32_t Ny, uint32_t Nz, bool xcd_
 }
 
 // Dynamic LDS per block that caps residency at `waves_per_simd` waves per SIMD (4 SIMDs
-// per CU), for a kernel that uses no LDS itself.  0 when the cap would not bind.
+// per CU), for a kernel that uses no LDS itself.  0 when the cap would not bind exactly.
+// A block's LDS is allocated in granules of 1/128 of the CU's LDS (1280 B of 160 KiB on
+// MI355X; measured: 53760 B per 256-thread block leaves 3 blocks per CU resident, 54272 B
+// only 2), so the size is a whole number of granules.
 inline size_t occupancy_lds(uint32_t waves_per_simd, uint32_t block_threads, size_t lds_per_cu, size_t lds_per_block_max) {
 	const uint32_t waves_per_block = (block_threads + 63u) / 64u;
 	const uint32_t blocks = (waves_per_simd * 4u) / waves_per_block;
-	if(blocks == 0u || lds_per_cu == 0u) return 0;
-	size_t lds = (lds_per_cu / blocks) & ~(size_t)255;
-	while(lds > 0 && lds_per_cu / lds < blocks) lds -= 256; // at least `blocks` fit
-	return lds <= lds_per_block_max ? lds : 0;
+	const size_t granule = lds_per_cu / 128u;
+	if(blocks == 0u || granule == 0u) return 0;
+	const size_t lds = (lds_per_cu / blocks) / granule * granule;
+	if(lds == 0u || lds > lds_per_block_max || lds_per_cu / lds != blocks) return 0;
+	return lds;
 }
 
 typedef void (*lattice_fn)(const LbmArgs&, const LaunchShape&, hipStream_t);
