@@ -96,11 +96,8 @@ template<> struct Storage<ST_FP16S> {
 // Decode: the code's exponent and mantissa fields placed at the binary32 positions give
 // 2^-112 times the value, for normal codes and (as a binary32 denormal) for denormal
 // codes alike; one exact multiplication by 2^112 finishes it (FP32 denormals are kept).
-// Encode is the reference's device rule (add 0x800, truncate; no saturation), written
-// branch-free: the normal code is the shifted bits minus the exponent bias difference,
-// masked like the reference's 4-bit field; the denormal shift is clamped at 31, which
-// yields the reference's zero for every exponent below its denormal range.  Both match
-// the previous, literal formulation on every input (all 2^16 codes, all 2^32 floats).
+// Encode is the reference's device rule (add 0x800, truncate; no saturation) in fewer
+// operations; it matches the literal rule on all 2^32 floats.
 template<> struct Storage<ST_FP16C> {
 	typedef uint16_t E;
 	__device__ static __forceinline__ float decode(uint16_t h) {
