@@ -78,15 +78,22 @@ constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIU
                              EXT_PRECISE_DENSITY | EXT_MOVING_BOUNDARIES | EXT_SUBGRID;
 
 // Measured launch tuning (see KernelSet), MI355X, 256^3 to 1536x768x768:
-// - the plain FP32 kernel (non-temporal DDF access) streams fastest with fewer concurrent
-//   DDF streams, 3 waves/SIMD, on the 3-D grid;
+// - FP32 kernels (non-temporal DDF access) stream fastest with fewer concurrent DDF
+//   streams: plain D3Q19/D3Q27 at 3 waves/SIMD (+11 % / +3 % over full occupancy),
+//   D3Q19 with EQUILIBRIUM_BOUNDARIES (+FORCE_FIELD) at 4 (+2-4 %), D3Q19 with
+//   VOLUME_FORCE (+POROSITY) at 3 (+4 %); the generic all-extension kernel is not capped;
 // - the 2-byte formats need full occupancy;
 // - FP16S gains 9-24 % on rows of several blocks from the XCD-mapped grid; FP32 loses up
 //   to 10 % there and FP16C, bound by its codec arithmetic, is indifferent;
 // - two or four cells per lane measured no better than one for any variant once all
 //   loads are issued ahead of the arithmetic.
 constexpr uint32_t tune_waves_cap(int Q, uint32_t ST, uint32_t EXT) {
-	return (EXT == 0u && ST == ST_FP32) ? 3u : 0u;
+	if(ST != ST_FP32) return 0u;
+	if(EXT == 0u) return 3u;
+	if(Q != 19) return 0u;
+	if(EXT == EXT_EQUILIBRIUM_BOUNDARIES || EXT == (EXT_EQUILIBRIUM_BOUNDARIES | EXT_FORCE_FIELD)) return 4u;
+	if(EXT == EXT_VOLUME_FORCE || EXT == (EXT_VOLUME_FORCE | EXT_POROSITY | EXT_PRECISE_DENSITY)) return 3u;
+	return 0u;
 }
 constexpr bool tune_xcd_map(int Q, uint32_t ST, uint32_t EXT) {
 	return ST == ST_FP16S;
