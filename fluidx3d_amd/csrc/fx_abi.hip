@@ -42,6 +42,7 @@ struct fx_ctx {
 	const fx::LbmKernels* k = nullptr;
 	fx::LaunchShape shape{};    // one lane per cell: every kernel but stream_collide
 	fx::LaunchShape shape_sc{}; // stream_collide (residency cap, XCD-mapped grid)
+	fx::LaunchShape shape_adj{}; // record pass and adjoint step (XCD-mapped grid)
 	double* partial = nullptr; // reduction partials (device)
 	uint64_t partial_cap = 0;  // in doubles
 	double* result = nullptr;  // 4 doubles (device)
@@ -409,6 +410,7 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 	ctx->k = fx::select_kernels(cfg->velocity_set, cfg->storage, cfg->collision == FX_TRT, cfg->extensions);
 	ctx->shape = fx::lattice_shape(cfg->Nx, cfg->Ny, cfg->Nz);
 	ctx->shape_sc = fx::stream_shape(cfg->Nx, cfg->Ny, cfg->Nz, ctx->k->stream_collide_xcd_map);
+	ctx->shape_adj = fx::adjoint_shape(cfg->Nx, cfg->Ny, cfg->Nz);
 	hipError_t e = hipSetDevice(device);
 	if(e == hipSuccess) {
 		hipDeviceProp_t prop;
@@ -716,7 +718,7 @@ int fx_lbm_stream_collide_record(fx_ctx* ctx, void* fi, float* rho, float* u, co
 	if(ctx->cfg.extensions & (FX_MOVING_BOUNDARIES | FX_SUBGRID))
 		return fail(FX_ERR_CONFIG, "the adjoint does not differentiate MOVING_BOUNDARIES or SUBGRID");
 	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
-	ctx->k->stream_collide_record(make_args(ctx, fi, rho, u, flags, t, fx_, fy_, fz_, F, alpha), ctx->shape, ctx->stream, moments);
+	ctx->k->stream_collide_record(make_args(ctx, fi, rho, u, flags, t, fx_, fy_, fz_, F, alpha), ctx->shape_adj, ctx->stream, moments);
 	return check_launch("stream_collide_record");
 }
 
@@ -741,7 +743,7 @@ int fx_lbm_adjoint_step(fx_ctx* ctx, float* lambda, const float* rho, const floa
 	if((ctx->cfg.extensions & FX_POROSITY) && !grad_alpha) return fail(FX_ERR_ARG, "POROSITY needs grad_alpha");
 	if((ctx->cfg.extensions & FX_EQUILIBRIUM_BOUNDARIES) && (!rho || !u)) return fail(FX_ERR_ARG, "TYPE_E presets rho/u required");
 	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
-	ctx->k->adjoint_step(adjoint_args(ctx, lambda, rho, u, flags, t, fx_, fy_, fz_, F, alpha, moments, plane_x, obj_weight, grad_alpha), ctx->shape, ctx->stream);
+	ctx->k->adjoint_step(adjoint_args(ctx, lambda, rho, u, flags, t, fx_, fy_, fz_, F, alpha, moments, plane_x, obj_weight, grad_alpha), ctx->shape_adj, ctx->stream);
 	return check_launch("adjoint_step");
 }
 
@@ -821,13 +823,13 @@ int fx_lbm_backward(fx_ctx* ctx, fx_adjoint* adj, const float* rho, const float*
 		FX_HIP(hipMemcpyAsync(adj->work_fi, adj->ckpt + s * adj->fi_bytes, adj->fi_bytes, hipMemcpyDeviceToDevice, ctx->stream));
 		for(uint64_t t = ts; t < te; t++) {
 			a.odd = (uint32_t)(t & 1u);
-			ctx->k->stream_collide_record(a, ctx->shape, ctx->stream, adj->moments + (t - ts) * 4u * N);
+			ctx->k->stream_collide_record(a, ctx->shape_adj, ctx->stream, adj->moments + (t - ts) * 4u * N);
 		}
 		for(uint64_t t = te; t-- > ts;) {
 			fx::AdjointArgs g = adjoint_args(ctx, adj->lambda, rho, u, flags, t, fx_, fy_, fz_, F, alpha, adj->moments + (t - ts) * 4u * N,
 			                                 c.plane_x, t >= c.window_begin ? wobj : 0.0f, grad_alpha);
 			g.a.ext &= ~FX_UPDATE_FIELDS;
-			ctx->k->adjoint_step(g, ctx->shape, ctx->stream);
+			ctx->k->adjoint_step(g, ctx->shape_adj, ctx->stream);
 		}
 		rc = check_launch("backward");
 		if(rc != FX_OK) return rc;

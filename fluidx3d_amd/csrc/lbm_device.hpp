@@ -564,9 +564,9 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
 // moments[c*N+n], c = 0..3 (16 B/cell): what the adjoint step needs of the forward state.
 // EXT is the context's extension subset; MOVING_BOUNDARIES and SUBGRID are not
 // differentiated and compiled out.
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT>
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide_record(LbmArgs a, float* moments) {
-	stream_collide_cell<Q, ST, TRT, EXT & ~(EXT_MOVING_BOUNDARIES | EXT_SUBGRID), true, false>(a, moments);
+	stream_collide_cell<Q, ST, TRT, EXT & ~(EXT_MOVING_BOUNDARIES | EXT_SUBGRID), true, XCD>(a, moments);
 }
 
 // update_fields (src/kernel.cpp:1794-1870)
@@ -731,12 +731,14 @@ struct AdjointArgs {
 	float obj_weight;        // objective weight of step t (0 outside the averaging window)
 };
 
-template<int Q, bool TRT>
+template<int Q, bool TRT, bool XCD = false>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_step(AdjointArgs g) {
 	const LbmArgs& a = g.a;
-	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+	uint32_t chunk, y, z;
+	if(!block_coords<XCD>(a, chunk, y, z)) return;
+	const uint32_t x = chunk * blockDim.x + threadIdx.x;
 	if(x >= a.Nx) return;
-	const Site s = make_site(x, blockIdx.y, blockIdx.z, a.Nx, a.Ny, a.Nz);
+	const Site s = make_site(x, y, z, a.Nx, a.Ny, a.Nz);
 	const uint64_t n = s.n();
 	const uint32_t flagsn = a.flags[n];
 	const uint32_t bo = flagsn & TYPE_BO;

@@ -51,6 +51,9 @@ inline size_t occupancy_lds(uint32_t waves_per_simd, uint32_t block_threads, siz
 	return lds;
 }
 
+// Grid of the adjoint kernels (record pass and adjoint step): XCD-mapped, full occupancy.
+inline LaunchShape adjoint_shape(uint32_t Nx, uint32_t Ny, uint32_t Nz) { return stream_shape(Nx, Ny, Nz, true); }
+
 typedef void (*lattice_fn)(const LbmArgs&, const LaunchShape&, hipStream_t);
 typedef void (*mass_fn)(const LbmArgs&, const LaunchShape&, hipStream_t, double*);
 typedef void (*record_fn)(const LbmArgs&, const LaunchShape&, hipStream_t, float*);
@@ -83,8 +86,9 @@ constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIU
 //   D3Q19 with EQUILIBRIUM_BOUNDARIES (+FORCE_FIELD) at 4 (+2-4 %), D3Q19 with
 //   VOLUME_FORCE (+POROSITY) at 3 (+4 %); the generic all-extension kernel is not capped;
 // - the 2-byte formats need full occupancy;
-// - FP16S gains 9-24 % on rows of several blocks from the XCD-mapped grid; FP32 loses up
-//   to 10 % there and FP16C, bound by its codec arithmetic, is indifferent;
+// - on rows of several blocks the XCD-mapped grid gains 9-24 % for FP16S and 6-10 % for
+//   FP32 (plain and porous, 384x192x192 to 768x384x192); FP16C is indifferent;
+// - one block per row (up to 768 threads) measured no better than chunks of 128/256;
 // - two or four cells per lane measured no better than one for any variant once all
 //   loads are issued ahead of the arithmetic.
 constexpr uint32_t tune_waves_cap(int Q, uint32_t ST, uint32_t EXT) {
@@ -96,7 +100,7 @@ constexpr uint32_t tune_waves_cap(int Q, uint32_t ST, uint32_t EXT) {
 	return 0u;
 }
 constexpr bool tune_xcd_map(int Q, uint32_t ST, uint32_t EXT) {
-	return ST == ST_FP16S;
+	return ST == ST_FP16S || ST == ST_FP32;
 }
 
 // Kernel set of one (velocity set, storage, collision) variant; EXT = 0 instantiates the
@@ -112,8 +116,18 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static void update_force_field(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_force_field<Q, ST><<<s.grid, s.block, 0, st>>>(a); }
 	static void mass_partial(const LbmArgs& a, const LaunchShape& s, hipStream_t st, double* p) { k_mass_partial<Q, ST><<<s.grid, s.block, 0, st>>>(a, p); }
 	static void update_moving_boundaries(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_moving_boundaries<Q><<<s.grid, s.block, 0, st>>>(a); }
-	static void stream_collide_record(const LbmArgs& a, const LaunchShape& s, hipStream_t st, float* m) { k_stream_collide_record<Q, ST, TRT, EXT><<<s.grid, s.block, 0, st>>>(a, m); }
-	static void adjoint_step(const AdjointArgs& g, const LaunchShape& s, hipStream_t st) { k_adjoint_step<Q, TRT><<<s.grid, s.block, 0, st>>>(g); }
+	// the adjoint kernels run on the XCD-mapped grid when rows take several blocks
+	// (adjoint_shape); measured at 384x192x192: record pass +14 %, adjoint step +10 %
+	static void stream_collide_record(const LbmArgs& a, const LaunchShape& s, hipStream_t st, float* m) {
+		LbmArgs b = a;
+		b.xchunks = s.xchunks;
+		k_stream_collide_record<Q, ST, TRT, EXT, true><<<s.grid, s.block, 0, st>>>(b, m);
+	}
+	static void adjoint_step(const AdjointArgs& g, const LaunchShape& s, hipStream_t st) {
+		AdjointArgs h = g;
+		h.a.xchunks = s.xchunks;
+		k_adjoint_step<Q, TRT, true><<<s.grid, s.block, 0, st>>>(h);
+	}
 	static void objective_partial(const LbmArgs& a, uint32_t plane_x, hipStream_t st, double* p) {
 		const uint64_t cells = (uint64_t)a.Ny * a.Nz;
 		k_objective_partial<Q, ST><<<(unsigned)((cells + 255u) / 256u), 256, 0, st>>>(a, plane_x, p);

@@ -30,12 +30,16 @@ def bits(names):
     return sum(EXT[n] for n in names)
 
 
+# 20x12x10: rows of one block; 300x4x3: rows of five 64-cell blocks (XCD-mapped grid)
+@pytest.mark.parametrize("grid", [(20, 12, 10), (300, 4, 3)], ids=lambda g: "x".join(map(str, g)))
 @pytest.mark.parametrize("q,collision", [(19, "SRT"), (19, "TRT"), (27, "SRT"), (27, "TRT")])
 @pytest.mark.parametrize("variant", ["porosity", "all"])
-def test_adjoint_step_matches_fp64_oracle(oracle, q, collision, variant):
+def test_adjoint_step_matches_fp64_oracle(oracle, q, collision, variant, grid):
+    """One record pass and one adjoint step against the FP64 oracle's adjoint step; the
+    record pass advances the DDFs exactly as a plain stream_collide does."""
     import fluidx3d_amd as fx
     from fluidx3d_amd import _abi
-    Nx, Ny, Nz = 20, 12, 10
+    Nx, Ny, Nz = grid
     N = Nx * Ny * Nz
     names = ("VOLUME_FORCE", "POROSITY") if variant == "porosity" else ("VOLUME_FORCE", "POROSITY", "FORCE_FIELD", "EQUILIBRIUM_BOUNDARIES", "PRECISE_DENSITY")
     rho, u, flags = scenes.random_scene(Nx, Ny, Nz, seed=5, solid_fraction=0.08, equilibrium_faces=variant == "all")
@@ -44,10 +48,13 @@ def test_adjoint_step_matches_fp64_oracle(oracle, q, collision, variant):
     F = (1e-4 * rng.uniform(-1, 1, 3 * N)).astype(np.float32)
     f = (2e-5, -1e-5, 5e-6)
     lbm = fx.LBM(Nx, Ny, Nz, 0.05, *f, velocity_set=q, collision=collision, extensions=names)
-    lbm.rho.host[:] = rho; lbm.u.host[:] = u; lbm.flags.host[:] = flags; lbm.porosity.host[:] = alpha
-    if lbm.F is not None:
-        lbm.F.host[:] = F
+    plain = fx.LBM(Nx, Ny, Nz, 0.05, *f, velocity_set=q, collision=collision, extensions=names)
+    for L_ in (lbm, plain):
+        L_.rho.host[:] = rho; L_.u.host[:] = u; L_.flags.host[:] = flags; L_.porosity.host[:] = alpha
+        if L_.F is not None:
+            L_.F.host[:] = F
     try:
+        plain.run(4)
         lbm.run(3)  # state before step t=3
         t = 3
         state = lbm.read_ddfs().astype(np.float64)
@@ -55,6 +62,7 @@ def test_adjoint_step_matches_fp64_oracle(oracle, q, collision, variant):
         moments = lbm.ctx.alloc(16 * N)
         _abi.check(L.fx_lbm_stream_collide_record(lbm.ctx.h, lbm.fi, lbm.rho.device, lbm.u.device, lbm.flags.device, t, *f,
                                                   lbm._ptr(lbm.F), lbm.porosity.device, moments))
+        assert np.array_equal(lbm.read_ddfs(), plain.read_ddfs()), "record pass differs from stream_collide"
         lam1 = rng.standard_normal(q * N).astype(np.float32)
         lam = lbm.ctx.alloc(4 * q * N)
         grad = lbm.ctx.alloc(4 * N)
@@ -80,10 +88,11 @@ def test_adjoint_step_matches_fp64_oracle(oracle, q, collision, variant):
         assert err < 2e-6 and gerr < 2e-5
     finally:
         lbm.close()
+        plain.close()
 
 
-def small_channel(fx_value=1e-5):
-    Nx, Ny, Nz = 32, 16, 12
+def small_channel(fx_value=1e-5, dims=(32, 16, 12)):
+    Nx, Ny, Nz = dims
     rho, u, flags, alpha = scenes.channel_c4(Nx, Ny, Nz)
     return (Nx, Ny, Nz), rho, u, flags, alpha
 
@@ -101,10 +110,12 @@ def gpu_gradient(dims, rho, u, flags, alpha, steps, window, plane, k, fxv=1e-5):
     return J, g
 
 
-def test_gradient_matches_fp64_adjoint(oracle):
-    """Porous channel 32x16x12, 300 steps, objective over the last 100: checkpointed GPU
-    gradient vs the FP64 oracle adjoint over the stored trajectory."""
-    dims, rho, u, flags, alpha = small_channel()
+# 32x16x12: rows of one block; 260x8x6: rows of five 64-cell blocks (XCD-mapped grid)
+@pytest.mark.parametrize("grid", [(32, 16, 12), (260, 8, 6)], ids=lambda g: "x".join(map(str, g)))
+def test_gradient_matches_fp64_adjoint(oracle, grid):
+    """Porous channel, 300 steps, objective over the last 100: checkpointed GPU gradient vs
+    the FP64 oracle adjoint over the stored trajectory."""
+    dims, rho, u, flags, alpha = small_channel(dims=grid)
     steps, window, plane = 300, 200, dims[0] - 1
     J, g = gpu_gradient(dims, rho, u, flags, alpha, steps, window, plane, 30)
     ref = oracle.OracleLBM(*dims, nu=1.0 / 6.0, storage="FP64", ext=bits(ADJ_EXT), fx=np.float32(1e-5))
