@@ -149,6 +149,52 @@ void main_setup() {
 #endif
 }
 
+#elif defined(SCENE_BOX_IO)
+// Box transfers of the field containers (read_from_device_1d/2d/3d, write_to_device_*):
+// the elements moved must be those of the reference's row loops (src/opencl.hpp:532-607),
+// per component and row: offset min(i*N+n, N*d), length min(x1-x0, N*d-offset).
+template<typename T> static ulong box_case(LBM& lbm, Memory_Container<T>& m, const bool write, const int form, const int dimension) {
+	const ulong N = m.length(), D = m.dimensions(), R = m.range(), Nx = lbm.get_Nx(), Ny = lbm.get_Ny(), Nz = lbm.get_Nz();
+	std::vector<T> device(R), host(R);
+	for(ulong e = 0ull; e < R; e++) { device[e] = (T)(e % 251ull + 1ull); host[e] = (T)((e * 7ull) % 249ull + 2ull); }
+	for(ulong e = 0ull; e < R; e++) m[e] = device[e];
+	m.write_to_device();
+	for(ulong e = 0ull; e < R; e++) m[e] = host[e];
+	// the box of each form (1-D rows run over component ends and the buffer end)
+	ulong x0 = 2ull, x1 = 9ull, y0 = 1ull, y1 = 6ull, z0 = 1ull, z1 = 4ull, nx = Nx, ny = Ny;
+	if(form == 1) { x0 = N - 20ull; x1 = N + 35ull; y0 = 0ull; y1 = 1ull; z0 = 0ull; z1 = 1ull; nx = N; ny = 1ull; }
+	if(form == 2) { x0 = 1ull; x1 = 11ull; y0 = 3ull; y1 = Ny * Nz + 1ull; z0 = 0ull; z1 = 1ull; ny = Ny * Nz; }
+	if(form == 1) write ? m.write_to_device_1d(x0, x1, dimension) : m.read_from_device_1d(x0, x1, dimension);
+	else if(form == 2) write ? m.write_to_device_2d(x0, x1, y0, y1, nx, ny, dimension) : m.read_from_device_2d(x0, x1, y0, y1, nx, ny, dimension);
+	else write ? m.write_to_device_3d(x0, x1, y0, y1, z0, z1, Nx, Ny, Nz, dimension) : m.read_from_device_3d(x0, x1, y0, y1, z0, z1, Nx, Ny, Nz, dimension);
+	std::vector<char> moved(R, 0);
+	for(ulong z = z0; z < z1; z++) for(ulong y = y0; y < y1; y++) {
+		const ulong n = x0 + (y + z * ny) * nx;
+		for(ulong i = dimension < 0 ? 0ull : (ulong)dimension; i < (dimension < 0 ? D : (ulong)dimension + 1ull); i++) {
+			const ulong off = min(i * N + n, R), len = min(x1 - x0, R - off);
+			for(ulong e = off; e < off + len; e++) moved[e] = 1;
+		}
+	}
+	ulong bad = 0ull;
+	if(write) {
+		for(ulong e = 0ull; e < R; e++) m[e] = (T)0;
+		m.read_from_device();
+		for(ulong e = 0ull; e < R; e++) bad += m[e] != (moved[e] ? host[e] : device[e]);
+	} else {
+		for(ulong e = 0ull; e < R; e++) bad += m[e] != (moved[e] ? device[e] : host[e]);
+	}
+	return bad;
+}
+void main_setup() {
+	LBM lbm(12u, 7u, 5u, 0.1f);
+	ulong bad = 0ull, cases = 0ull;
+	for(int write = 0; write < 2; write++) for(int form = 0; form < 3; form++) {
+		for(int dimension = -1; dimension < 3; dimension++) { bad += box_case(lbm, lbm.u, write != 0, form, dimension); cases++; }
+		bad += box_case(lbm, lbm.flags, write != 0, form, -1); cases++;
+	}
+	println("box transfers: " + to_string(cases) + " cases, " + to_string(bad) + " wrong elements");
+}
+
 #else
 void main_setup() {
 	print_error("no scene selected: compile with -DBENCHMARK, -DSCENE_TAYLOR_GREEN, -DSCENE_SPHERE, -DSCENE_CAVITY or -DSCENE_POROUS_CHANNEL");

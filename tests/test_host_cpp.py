@@ -173,3 +173,14 @@ def test_cpp_lid_driven_cavity_matches_python_path(tmp_path):
     assert np.array_equal(u_cpp, u_py)
     below_lid = (z == L - 2) & (x > 0) & (x < L - 1) & (y > 2) & (y < L - 3)
     assert u_py[below_lid, 1].mean() > 0.02
+
+
+@pytest.mark.gpu
+def test_cpp_box_transfers_move_the_reference_elements(tmp_path):
+    """Memory_Container::read_from_device_1d/2d/3d and write_to_device_1d/2d/3d (the
+    reference's Memory<T> box transfers, src/opencl.hpp:532-607) on u (3 components, each
+    and all) and flags: every element that the reference's row loops move, and no other,
+    changes (the scene checks each element itself)."""
+    r = subprocess.run([binary("box_io")], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "box transfers: 30 cases, 0 wrong elements" in r.stdout, r.stdout
