@@ -109,14 +109,16 @@ template<> struct Storage<ST_FP16C> {
 	// e_b mod 16, so the normal magnitude is bits 12..26 of b.  The reference's denormal
 	// code ((0x7FF800 + m_b) >> (124 - e_b) + 1) >> 1 equals floor(|x|*2^25 + 1/2) for every
 	// |x| below 2^-14*(1 - 2^-12) (e_b <= 112; zero from e_b <= 100 on), with or without a
-	// carry out of the mantissa in b, and is taken as (floor(|x|*2^26) + 1) >> 1: the
-	// product is exact and the conversion truncates, so no rounding enters.  A NaN fails the
-	// comparison and takes the normal path, as in the reference.
+	// carry out of the mantissa in b.  It is taken as the truncating conversion of
+	// fma(|x|, 2^25, 1/2 - 2^-25): the single rounding of the fma turns the ties
+	// |x|*2^25 = k - 1/2 into k (ties to even) and keeps every value below an integer below
+	// it (checked on all 2^32 floats; the addend 1/2 itself fails at |x|*2^25 = 1/2 - 2^-25).
+	// A NaN fails the comparison and takes the normal path, as in the reference.
 	__device__ static __forceinline__ uint16_t encode(float x) {
 		const uint32_t b = __float_as_uint(x) + 0x00000800u;
 		const uint32_t normal = (b >> 12) & 0x7FFFu;
 		const float ax = fabsf(x);
-		const uint32_t denormal = (__float2uint_rz(ax * 0x1p26f) + 1u) >> 1;
+		const uint32_t denormal = __float2uint_rz(__builtin_fmaf(ax, 0x1p25f, 0x1.fffffep-2f));
 		const uint32_t mag = ax < __uint_as_float(0x387FF800u) ? denormal : normal;
 		return (uint16_t)(((b >> 16) & 0x8000u) | mag);
 	}
