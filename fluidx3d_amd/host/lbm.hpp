@@ -175,6 +175,19 @@ public:
 	}
 	ulong index(const uint x, const uint y, const uint z) const { return (ulong)x + ((ulong)y + (ulong)z * (ulong)Ny) * (ulong)Nx; }
 	ulong index(const uint3 xyz) const { return index(xyz.x, xyz.y, xyz.z); }
+	// position -> closest grid cell, positions wrapped into the periodic box
+	// (src/lbm.hpp:484-500, 509-515)
+	float3 mirror_position(const float3& p) const {
+		auto wrap = [](const float v, const float n) { return (v >= 0.0f ? 1.0f : -1.0f) * (std::fmod(std::fabs(v) + 0.5f * n, n) - 0.5f * n); };
+		return float3(wrap(p.x, (float)Nx), wrap(p.y, (float)Ny), wrap(p.z, (float)Nz));
+	}
+	void coordinates(const float3& p, uint& x, uint& y, uint& z) const {
+		const float3 mp = mirror_position(p);
+		x = (uint)(mp.x + 1.5f * (float)Nx) % Nx;
+		y = (uint)(mp.y + 1.5f * (float)Ny) % Ny;
+		z = (uint)(mp.z + 1.5f * (float)Nz) % Nz;
+	}
+	ulong index(const float3& p) const { uint x = 0u, y = 0u, z = 0u; coordinates(p, x, y, z); return index(x, y, z); }
 	float3 position(const uint x, const uint y, const uint z) const { return float3((float)x + 0.5f - 0.5f * (float)Nx, (float)y + 0.5f - 0.5f * (float)Ny, (float)z + 0.5f - 0.5f * (float)Nz); }
 	float3 position(const ulong n) const { uint x, y, z; coordinates(n, x, y, z); return position(x, y, z); }
 	float3 relative_position(const uint x, const uint y, const uint z) const { return float3(((float)x + 0.5f) / (float)Nx - 0.5f, ((float)y + 0.5f) / (float)Ny - 0.5f, ((float)z + 0.5f) / (float)Nz - 0.5f); }

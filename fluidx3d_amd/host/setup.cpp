@@ -193,6 +193,14 @@ void main_setup() {
 		bad += box_case(lbm, lbm.flags, write != 0, form, -1); cases++;
 	}
 	println("box transfers: " + to_string(cases) + " cases, " + to_string(bad) + " wrong elements");
+	// position -> cell: every cell centre maps back to its cell, also shifted by whole periods
+	ulong wrong = 0ull;
+	for(ulong n = 0ull; n < lbm.get_N(); n++) {
+		const float3 p = lbm.position(n);
+		wrong += lbm.index(p) != n;
+		wrong += lbm.index(float3(p.x + (float)lbm.get_Nx(), p.y - 2.0f * (float)lbm.get_Ny(), p.z + 3.0f * (float)lbm.get_Nz())) != n;
+	}
+	println("positions: " + to_string(wrong) + " wrong cells");
 }
 
 #else

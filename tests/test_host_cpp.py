@@ -184,3 +184,5 @@ def test_cpp_box_transfers_move_the_reference_elements(tmp_path):
     r = subprocess.run([binary("box_io")], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "box transfers: 30 cases, 0 wrong elements" in r.stdout, r.stdout
+    # LBM::index(float3) / coordinates(float3) / mirror_position (src/lbm.hpp:484-515)
+    assert "positions: 0 wrong cells" in r.stdout, r.stdout
