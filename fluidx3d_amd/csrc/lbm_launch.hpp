@@ -86,8 +86,9 @@ constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIU
 //   D3Q19 with EQUILIBRIUM_BOUNDARIES (+FORCE_FIELD) at 4 (+2-4 %), D3Q19 with
 //   VOLUME_FORCE (+POROSITY) at 3 (+4 %); the generic all-extension kernel is not capped;
 // - the 2-byte formats need full occupancy;
-// - on rows of several blocks the XCD-mapped grid gains 9-24 % for FP16S and 6-10 % for
-//   FP32 (plain and porous, 384x192x192 to 768x384x192); FP16C is indifferent;
+// - on rows of several blocks the XCD-mapped grid gains 9-24 % for FP16S, 6-10 % for FP32
+//   (plain and porous, 384x192x192 to 768x384x192) and 5-10 % for FP16C (512x256x256,
+//   plain D3Q19 and the D3Q27 TRT EQUILIBRIUM_BOUNDARIES + FORCE_FIELD kernel);
 // - one block per row (up to 768 threads) measured no better than chunks of 128/256;
 // - two or four cells per lane measured no better than one for any variant once all
 //   loads are issued ahead of the arithmetic.
@@ -100,7 +101,7 @@ constexpr uint32_t tune_waves_cap(int Q, uint32_t ST, uint32_t EXT) {
 	return 0u;
 }
 constexpr bool tune_xcd_map(int Q, uint32_t ST, uint32_t EXT) {
-	return ST == ST_FP16S || ST == ST_FP32;
+	return true;
 }
 
 // Kernel set of one (velocity set, storage, collision) variant; EXT = 0 instantiates the
