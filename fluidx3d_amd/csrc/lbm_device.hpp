@@ -63,20 +63,20 @@ template<typename E> __device__ __forceinline__ E* elem(E* p, uint32_t o) {
 }
 
 template<uint32_t ST> struct Storage;
-// FP32 DDFs are streamed with non-temporal loads and stores: no DDF value is touched
-// twice in a step, and on MI355X this measured faster for 4-byte DDFs (while it is slower
-// for the 2-byte formats, which keep regular loads).
+// DDFs are streamed with non-temporal loads and stores: no DDF value is touched twice in
+// a step.  Slots at x-shifted neighbour positions are the exception: a wave's access there
+// straddles a cache line that the neighbouring wave shares, and ordinary (cached) stores
+// let the two halves meet in L2 instead of being written back in two non-temporal pieces.
+// Measured at 256^3: FP32 D3Q19 non-temporal against cached at the unshifted slots +11 %,
+// cached x-shifted stores +4.7 %; D3Q27 also keeps cached x-shifted loads (+3 %); FP16S
+// +9 % from non-temporal loads and stores together at the unshifted slots (either alone
+// gains nothing).
 template<> struct Storage<ST_FP32> {
 	typedef float E;
 	__device__ static __forceinline__ float raw(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
 	__device__ static __forceinline__ float decode(float x) { return x; }
 	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return raw(p, o); }
 	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { __builtin_nontemporal_store(x, elem(p, o)); }
-	// Slots at x-shifted neighbour positions: a wave's access there straddles a cache line
-	// that the neighbouring wave shares.  Ordinary (cached) stores let the two halves meet
-	// in L2 instead of being written back in two non-temporal pieces (measured +4.7 % for
-	// D3Q19, where ordinary loads there measured slower, and, together with ordinary loads,
-	// +11 % for D3Q27).
 	template<int Q> __device__ static __forceinline__ float raw_shifted(const E* p, uint32_t o) { return Q == 27 ? *elem(p, o) : raw(p, o); }
 	template<int Q> __device__ static __forceinline__ void store_shifted(E* p, uint32_t o, float x) { *elem(p, o) = x; }
 };
@@ -86,11 +86,11 @@ template<> struct Storage<ST_FP16S> {
 	typedef uint16_t E;
 	__device__ static __forceinline__ float decode(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h) * 3.0517578E-5f; }
 	__device__ static __forceinline__ uint16_t encode(float x) { return __builtin_bit_cast(uint16_t, (_Float16)(x * 32768.0f)); }
-	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return *elem(p, o); }
-	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(*elem(p, o)); }
-	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { *elem(p, o) = encode(x); }
-	template<int Q> __device__ static __forceinline__ uint16_t raw_shifted(const E* p, uint32_t o) { return raw(p, o); }
-	template<int Q> __device__ static __forceinline__ void store_shifted(E* p, uint32_t o, float x) { store(p, o, x); }
+	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
+	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(raw(p, o)); }
+	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { __builtin_nontemporal_store(encode(x), elem(p, o)); }
+	template<int Q> __device__ static __forceinline__ uint16_t raw_shifted(const E* p, uint32_t o) { return *elem(p, o); }
+	template<int Q> __device__ static __forceinline__ void store_shifted(E* p, uint32_t o, float x) { *elem(p, o) = encode(x); }
 };
 // FP16C: 1 sign, 4 exponent (bias 15), 11 mantissa bits (src/kernel.cpp:848-859).
 // Decode: the code's exponent and mantissa fields placed at the binary32 positions give
@@ -122,11 +122,11 @@ template<> struct Storage<ST_FP16C> {
 		const uint32_t mag = ax < __uint_as_float(0x387FF800u) ? denormal : normal;
 		return (uint16_t)(((b >> 16) & 0x8000u) | mag);
 	}
-	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return *elem(p, o); }
-	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(*elem(p, o)); }
-	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { *elem(p, o) = encode(x); }
-	template<int Q> __device__ static __forceinline__ uint16_t raw_shifted(const E* p, uint32_t o) { return raw(p, o); }
-	template<int Q> __device__ static __forceinline__ void store_shifted(E* p, uint32_t o, float x) { store(p, o, x); }
+	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
+	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(raw(p, o)); }
+	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { __builtin_nontemporal_store(encode(x), elem(p, o)); }
+	template<int Q> __device__ static __forceinline__ uint16_t raw_shifted(const E* p, uint32_t o) { return *elem(p, o); }
+	template<int Q> __device__ static __forceinline__ void store_shifted(E* p, uint32_t o, float x) { *elem(p, o) = encode(x); }
 };
 
 // ------------------------------------------------------------------------------------

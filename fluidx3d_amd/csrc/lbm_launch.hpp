@@ -85,7 +85,7 @@ constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIU
 //   streams: plain D3Q19/D3Q27 at 3 waves/SIMD (+11 % / +3 % over full occupancy),
 //   D3Q19 with EQUILIBRIUM_BOUNDARIES (+FORCE_FIELD) at 4 (+2-4 %), D3Q19 with
 //   VOLUME_FORCE (+POROSITY) at 3 (+4 %); the generic all-extension kernel is not capped;
-// - the 2-byte formats need full occupancy;
+// - the 2-byte formats (non-temporal too) need full occupancy;
 // - on rows of several blocks the XCD-mapped grid gains 9-24 % for FP16S, 6-10 % for FP32
 //   (plain and porous, 384x192x192 to 768x384x192) and 5-10 % for FP16C (512x256x256,
 //   plain D3Q19 and the D3Q27 TRT EQUILIBRIUM_BOUNDARIES + FORCE_FIELD kernel);
