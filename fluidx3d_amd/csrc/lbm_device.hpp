@@ -61,6 +61,16 @@ template<typename E> __device__ __forceinline__ const E* elem(const E* p, uint32
 template<typename E> __device__ __forceinline__ E* elem(E* p, uint32_t o) {
 	return (E*)((char*)p + (uint32_t)(o * (uint32_t)sizeof(E)));
 }
+// A row base that is the same in every lane of the wave, passed through readfirstlane: the
+// compiler then keeps it in scalar registers, and the load and the later store of the same
+// element do not share a 64-bit per-lane address held in vector registers across the
+// collision.  Only for bases built from the block's row (never from a per-lane cell index).
+template<typename E> __device__ __forceinline__ E* uniform(E* p) {
+	const uint64_t v = (uint64_t)p;
+	const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+	return (E*)(((uint64_t)hi << 32) | lo);
+}
+template<bool U, typename E> __device__ __forceinline__ E* row_base(E* p) { if constexpr(U) return uniform(p); else return p; }
 
 template<uint32_t ST> struct Storage;
 // DDFs are streamed with non-temporal loads and stores: no DDF value is touched twice in
@@ -156,34 +166,46 @@ template<int Q> __device__ __forceinline__ constexpr float weight(int i) {
 // ------------------------------------------------------------------------------------
 __device__ __forceinline__ float clampc(float x) { return fminf(fmaxf(x, -DEF_C), DEF_C); } // OpenCL clamp
 
-__device__ __forceinline__ void eq_pair(float* feq, int i, float rw, float rm1w, float uk, float c3) {
-	const float a = __builtin_fmaf(uk, uk, c3);
-	feq[i] = __builtin_fmaf(rw, __builtin_fmaf(0.5f, a, uk), rm1w);
-	feq[i + 1] = __builtin_fmaf(rw, __builtin_fmaf(0.5f, a, -uk), rm1w);
-}
 // calculate_f_eq (src/kernel.cpp:1004-1061), DDF-shifted equilibrium; rhom1 = rho-1 (the
-// reference forms it as rho - 1.0f; with PRECISE_DENSITY it is the DDF sum itself)
-template<int Q> __device__ __forceinline__ void calc_feq(float rho, float rhom1, float ux, float uy, float uz, float* feq) {
-	const float c3 = -3.0f * ((ux * ux + uy * uy) + uz * uz);
-	uz *= 3.0f; ux *= 3.0f; uy *= 3.0f;
-	feq[0] = VS<Q>::w0 * __builtin_fmaf(rho, 0.5f * c3, rhom1);
-	const float rs = VS<Q>::ws * rho, re = VS<Q>::we * rho, rm1s = VS<Q>::ws * rhom1, rm1e = VS<Q>::we * rhom1;
-	eq_pair(feq, 1, rs, rm1s, ux, c3);
-	eq_pair(feq, 3, rs, rm1s, uy, c3);
-	eq_pair(feq, 5, rs, rm1s, uz, c3);
-	eq_pair(feq, 7, re, rm1e, ux + uy, c3);
-	eq_pair(feq, 9, re, rm1e, ux + uz, c3);
-	eq_pair(feq, 11, re, rm1e, uy + uz, c3);
-	eq_pair(feq, 13, re, rm1e, ux - uy, c3);
-	eq_pair(feq, 15, re, rm1e, ux - uz, c3);
-	eq_pair(feq, 17, re, rm1e, uy - uz, c3);
-	if constexpr(Q == 27) {
-		const float rc = VS<Q>::wc * rho, rm1c = VS<Q>::wc * rhom1;
-		eq_pair(feq, 19, rc, rm1c, (ux + uy) + uz, c3);
-		eq_pair(feq, 21, rc, rm1c, (ux + uy) - uz, c3);
-		eq_pair(feq, 23, rc, rm1c, (ux - uy) + uz, c3);
-		eq_pair(feq, 25, rc, rm1c, (-ux + uy) + uz, c3);
+// reference forms it as rho - 1.0f; with PRECISE_DENSITY it is the DDF sum itself).  The
+// shared terms are formed once; pair(k) gives the two populations of the opposite pair
+// (k, k+1), so a collision can take them one pair at a time.
+template<int Q> struct Equilibrium {
+	float c3, ux, uy, uz, f0, rs, re, rc, rm1s, rm1e, rm1c;
+	__device__ __forceinline__ Equilibrium(float rho, float rhom1, float vx, float vy, float vz) {
+		c3 = -3.0f * ((vx * vx + vy * vy) + vz * vz);
+		uz = vz * 3.0f; ux = vx * 3.0f; uy = vy * 3.0f;
+		f0 = VS<Q>::w0 * __builtin_fmaf(rho, 0.5f * c3, rhom1);
+		rs = VS<Q>::ws * rho; re = VS<Q>::we * rho; rm1s = VS<Q>::ws * rhom1; rm1e = VS<Q>::we * rhom1;
+		rc = VS<Q>::wc * rho; rm1c = VS<Q>::wc * rhom1;
 	}
+	__device__ __forceinline__ void pair(int k, float& a, float& b) const {
+		float r = re, m = rm1e, uk;
+		switch(k) {
+			case 1: r = rs; m = rm1s; uk = ux; break;
+			case 3: r = rs; m = rm1s; uk = uy; break;
+			case 5: r = rs; m = rm1s; uk = uz; break;
+			case 7: uk = ux + uy; break;
+			case 9: uk = ux + uz; break;
+			case 11: uk = uy + uz; break;
+			case 13: uk = ux - uy; break;
+			case 15: uk = ux - uz; break;
+			case 17: uk = uy - uz; break;
+			case 19: r = rc; m = rm1c; uk = (ux + uy) + uz; break;
+			case 21: r = rc; m = rm1c; uk = (ux + uy) - uz; break;
+			case 23: r = rc; m = rm1c; uk = (ux - uy) + uz; break;
+			default: r = rc; m = rm1c; uk = (-ux + uy) + uz; break; // 25
+		}
+		const float t = __builtin_fmaf(uk, uk, c3);
+		a = __builtin_fmaf(r, __builtin_fmaf(0.5f, t, uk), m);
+		b = __builtin_fmaf(r, __builtin_fmaf(0.5f, t, -uk), m);
+	}
+};
+template<int Q> __device__ __forceinline__ void calc_feq(float rho, float rhom1, float ux, float uy, float uz, float* feq) {
+	const Equilibrium<Q> eq(rho, rhom1, ux, uy, uz);
+	feq[0] = eq.f0;
+#pragma unroll
+	for(int k = 1; k < Q; k += 2) eq.pair(k, feq[k], feq[k + 1]);
 }
 
 // calculate_rho_u (src/kernel.cpp:1063-1088): rho summed in slot order, +1 last (DDF
@@ -222,14 +244,23 @@ template<int Q> __device__ __forceinline__ float cdot(int i, float x, float y, f
 	if(c != 0) { s = any ? (c > 0 ? s + z : s - z) : (c > 0 ? z : -z); }
 	return s;
 }
-// calculate_forcing_terms (src/kernel.cpp:1090-1101), Guo forcing
-template<int Q> __device__ __forceinline__ void calc_forcing(float ux, float uy, float uz, float fx, float fy, float fz, float* Fin) {
-	const float uF = -THIRD * __builtin_fmaf(ux, fx, __builtin_fmaf(uy, fy, uz * fz));
-	Fin[0] = (9.0f * VS<Q>::w0) * uF;
-#pragma unroll
-	for(int i = 1; i < Q; i++) {
-		Fin[i] = (9.0f * weight<Q>(i)) * __builtin_fmaf(cdot<Q>(i, fx, fy, fz), cdot<Q>(i, ux, uy, uz) + THIRD, uF);
+// calculate_forcing_terms (src/kernel.cpp:1090-1101), Guo forcing; at(i) is population i's
+// term, so a collision can take the terms one pair at a time.
+template<int Q> struct Forcing {
+	float uF, ux, uy, uz, fx, fy, fz;
+	__device__ __forceinline__ Forcing(float vx, float vy, float vz, float gx, float gy, float gz)
+		: ux(vx), uy(vy), uz(vz), fx(gx), fy(gy), fz(gz) {
+		uF = -THIRD * __builtin_fmaf(ux, fx, __builtin_fmaf(uy, fy, uz * fz));
 	}
+	__device__ __forceinline__ float at(int i) const {
+		if(i == 0) return (9.0f * VS<Q>::w0) * uF;
+		return (9.0f * weight<Q>(i)) * __builtin_fmaf(cdot<Q>(i, fx, fy, fz), cdot<Q>(i, ux, uy, uz) + THIRD, uF);
+	}
+};
+template<int Q> __device__ __forceinline__ void calc_forcing(float ux, float uy, float uz, float fx, float fy, float fz, float* Fin) {
+	const Forcing<Q> fo(ux, uy, uz, fx, fy, fz);
+#pragma unroll
+	for(int i = 0; i < Q; i++) Fin[i] = fo.at(i);
 }
 
 // Force block of stream_collide/update_fields (src/kernel.cpp:1549-1563, 1847-1859)
@@ -287,40 +318,42 @@ __device__ __forceinline__ Site make_site(uint32_t x, uint32_t y, uint32_t z, ui
 // slot pair (n, i|i+1) and (n+e_i, i+1|i); which one holds which population flips with
 // the parity of t.  Loads and stores touch the same locations, so the update is in place
 // and race-free.
-template<int Q, uint32_t ST>
+// U: the row offsets in s are the block's (wave-uniform); false where a lane has a row of
+// its own (the objective plane's one lane per (y, z)).
+template<int Q, uint32_t ST, bool U = false>
 __device__ __forceinline__ void load_raw(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, typename Storage<ST>::E* r) {
 	typedef Storage<ST> S;
 	const uint64_t n0 = s.R[4];
-	r[0] = S::raw(fi + n0, s.x);
+	r[0] = S::raw(row_base<U>(fi + n0), s.x);
 #pragma unroll
 	for(int i = 1; i < Q; i += 2) {
 		const uint64_t sa = odd ? (uint64_t)i : (uint64_t)(i + 1);
 		const uint64_t sb = odd ? (uint64_t)(i + 1) : (uint64_t)i;
-		r[i] = S::raw(fi + sa * N + n0, s.x);
-		const typename Storage<ST>::E* pb = fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]);
+		r[i] = S::raw(row_base<U>(fi + sa * N + n0), s.x);
+		const typename Storage<ST>::E* pb = row_base<U>(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]));
 		r[i + 1] = VS<Q>::cx[i] != 0 ? S::template raw_shifted<Q>(pb, s.X[VS<Q>::cx[i] + 1]) : S::raw(pb, s.x);
 	}
 }
-template<int Q, uint32_t ST>
+template<int Q, uint32_t ST, bool U = false>
 __device__ __forceinline__ void load_f(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, float* f) {
 	typename Storage<ST>::E r[Q];
-	load_raw<Q, ST>(fi, N, s, odd, r);
+	load_raw<Q, ST, U>(fi, N, s, odd, r);
 #pragma unroll
 	for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
 }
-template<int Q, uint32_t ST>
+template<int Q, uint32_t ST, bool U = false>
 __device__ __forceinline__ void store_f(typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, const float* f) {
 	typedef Storage<ST> S;
 	const uint64_t n0 = s.R[4];
-	S::store(fi + n0, s.x, f[0]);
+	S::store(row_base<U>(fi + n0), s.x, f[0]);
 #pragma unroll
 	for(int i = 1; i < Q; i += 2) {
 		const uint64_t sa = odd ? (uint64_t)i : (uint64_t)(i + 1);
 		const uint64_t sb = odd ? (uint64_t)(i + 1) : (uint64_t)i;
-		typename Storage<ST>::E* pb = fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]);
+		typename Storage<ST>::E* pb = row_base<U>(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]));
 		if(VS<Q>::cx[i] != 0) S::template store_shifted<Q>(pb, s.X[VS<Q>::cx[i] + 1], f[i]);
 		else S::store(pb, s.x, f[i]);
-		S::store(fi + sa * N + n0, s.x, f[i + 1]);
+		S::store(row_base<U>(fi + sa * N + n0), s.x, f[i + 1]);
 	}
 }
 
@@ -427,6 +460,74 @@ template<int Q> __device__ __forceinline__ float subgrid_rate(float w, const flo
 //
 // collide_cell: everything between load_f and store_f for the fluid or TYPE_E cell n;
 // fhn holds the streamed-in DDFs on entry and the post-collision DDFs on return.
+// SRT/TRT collision (src/kernel.cpp:1595-1633) one opposite pair at a time: each pair's
+// equilibrium and forcing terms are formed right where the pair collides, the same
+// operations as with whole arrays, so only a pair's values are live at once.  Used where it
+// measured faster (256^3, same process): the forced kernels (VOLUME_FORCE, POROSITY) of
+// D3Q19 and the forced TRT kernels of D3Q27, +10-14 % for the 16-bit TRT ones; elsewhere
+// the whole-array form was as fast or faster (D3Q27 FP16C TRT EQUILIBRIUM_BOUNDARIES: -7 %
+// by pairs).  SUBGRID always takes whole arrays.
+constexpr bool collide_by_pairs(int Q, bool TRT, uint32_t EXT) {
+	return (EXT & EXT_SUBGRID) == 0u && (EXT & (EXT_VOLUME_FORCE | EXT_POROSITY)) != 0u && (TRT || Q == 19);
+}
+// Uniform row bases (readfirstlane) for stream_collide: the forced D3Q19 16-bit kernels
+// drop from 96-99 to 58-66 VGPRs and gain 2-20 %; D3Q27 (27 slot bases in scalar
+// registers) and the unforced kernels lost up to 30 %, so they keep the plain bases.
+constexpr bool uniform_rows(int Q, uint32_t ST, uint32_t EXT) {
+	return Q == 19 && ST != ST_FP32 && (EXT & (EXT_VOLUME_FORCE | EXT_POROSITY)) != 0u;
+}
+template<int Q, bool TRT>
+__device__ __forceinline__ void collide_pairs(float w, bool vf, bool isE, const Equilibrium<Q>& eq, const Forcing<Q>& fo, float* fhn) {
+	if constexpr(!TRT) { // src/kernel.cpp:1595-1604
+		const float c_tau = __builtin_fmaf(w, -0.5f, 1.0f);
+		const float omw = 1.0f - w;
+		auto one = [&](int i, float fe) {
+			const float Fi = vf ? fo.at(i) * c_tau : 0.0f;
+			const float c = __builtin_fmaf(omw, fhn[i], __builtin_fmaf(w, fe, Fi));
+			fhn[i] = isE ? fe : c;
+		};
+		one(0, eq.f0);
+#pragma unroll
+		for(int k = 1; k < Q; k += 2) {
+			float ea, eb;
+			eq.pair(k, ea, eb);
+			one(k, ea);
+			one(k + 1, eb);
+		}
+	} else { // src/kernel.cpp:1605-1633, Lambda = 3/16
+		const float wp = w;
+		const float wm = 1.0f / (0.1875f / (1.0f / w - 0.5f) + 0.5f);
+		const float c_taup = __builtin_fmaf(wp, -0.25f, 0.5f), c_taum = __builtin_fmaf(wm, -0.25f, 0.5f);
+		const float hwp = 0.5f * wp, hwm = 0.5f * wm;
+		auto trt = [&](float fe, float feb, float fh, float fhb, float F) {
+			return __builtin_fmaf(hwp, ((fe - fh) + feb) - fhb, __builtin_fmaf(hwm, ((fe - feb) - fh) + fhb, fh + F));
+		};
+		{
+			float F0 = 0.0f;
+			if(vf) {
+				const float g = fo.at(0);
+				F0 = __builtin_fmaf(c_taup, g + g, c_taum * (g - g));
+			}
+			const float c = trt(eq.f0, eq.f0, fhn[0], fhn[0], F0);
+			fhn[0] = isE ? eq.f0 : c;
+		}
+#pragma unroll
+		for(int k = 1; k < Q; k += 2) {
+			float ea, eb;
+			eq.pair(k, ea, eb);
+			float Fa = 0.0f, Fb = 0.0f;
+			if(vf) {
+				const float ga = fo.at(k), gb = fo.at(k + 1);
+				Fa = __builtin_fmaf(c_taup, ga + gb, c_taum * (ga - gb));
+				Fb = __builtin_fmaf(c_taup, gb + ga, c_taum * (gb - ga));
+			}
+			const float ca = trt(ea, eb, fhn[k], fhn[k + 1], Fa);
+			const float cb = trt(eb, ea, fhn[k + 1], fhn[k], Fb);
+			fhn[k] = isE ? ea : ca;
+			fhn[k + 1] = isE ? eb : cb;
+		}
+	}
+}
 template<int Q, bool TRT, uint32_t EXT, bool RECORD>
 __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint32_t bo, float* fhn, float* moments) {
 	const bool isE = (a.ext & EXT_EQUILIBRIUM_BOUNDARIES) && bo == TYPE_E;
@@ -453,14 +554,19 @@ __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint3
 	const float alphan = (a.ext & EXT_POROSITY) ? a.alpha[n] : 0.0f;
 	force_block(a.ext, rhon, uxn, uyn, uzn, fxn, fyn, fzn, alphan);
 	const bool vf = a.ext & (EXT_VOLUME_FORCE | EXT_POROSITY);
+	if((a.ext & EXT_UPDATE_FIELDS) && !isE) {
+		a.rho[n] = rhon; a.u[n] = uxn; a.u[a.N + n] = uyn; a.u[2ull * a.N + n] = uzn;
+	}
+	if constexpr(collide_by_pairs(Q, TRT, EXT)) {
+		collide_pairs<Q, TRT>(a.w, vf, isE, Equilibrium<Q>(rhon, rhom1, uxn, uyn, uzn), Forcing<Q>(uxn, uyn, uzn, fxn, fyn, fzn), fhn);
+		return;
+	}
+	// whole arrays (SUBGRID needs every equilibrium population before the collision)
 	float Fin[Q];
 	if(vf) calc_forcing<Q>(uxn, uyn, uzn, fxn, fyn, fzn, Fin);
 	else {
 #pragma unroll
 		for(int i = 0; i < Q; i++) Fin[i] = 0.0f;
-	}
-	if((a.ext & EXT_UPDATE_FIELDS) && !isE) {
-		a.rho[n] = rhon; a.u[n] = uxn; a.u[a.N + n] = uyn; a.u[2ull * a.N + n] = uzn;
 	}
 	float feq[Q];
 	calc_feq<Q>(rhon, rhom1, uxn, uyn, uzn, feq);
@@ -545,16 +651,16 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 		// every load is issued before the first decode: left alone, the scheduler may
 		// convert each 16-bit value right after its load and wait for it, one at a time
 		E r[Q];
-		load_raw<Q, ST>(fi, a.N, s, a.odd, r);
+		load_raw<Q, ST, uniform_rows(Q, ST, EXT)>(fi, a.N, s, a.odd, r);
 		__builtin_amdgcn_sched_barrier(0);
 #pragma unroll
 		for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
 	} else {
-		load_f<Q, ST>(fi, a.N, s, a.odd, f);
+		load_f<Q, ST, uniform_rows(Q, ST, EXT)>(fi, a.N, s, a.odd, f);
 	}
 	if((a.ext & EXT_MOVING_BOUNDARIES) && bo == TYPE_MS) moving_boundaries<Q>(a, s, f);
 	collide_cell<Q, TRT, EXT, RECORD>(a, n, bo, f, moments);
-	store_f<Q, ST>(fi, a.N, s, a.odd, f);
+	store_f<Q, ST, uniform_rows(Q, ST, EXT)>(fi, a.N, s, a.odd, f);
 }
 // XCD: the kernel may run on the 1-D XCD-mapped grid (a.xchunks > 0); false compiles the
 // mapping out.
