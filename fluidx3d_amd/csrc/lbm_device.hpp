@@ -471,10 +471,13 @@ constexpr bool collide_by_pairs(int Q, bool TRT, uint32_t EXT) {
 	return (EXT & EXT_SUBGRID) == 0u && (EXT & (EXT_VOLUME_FORCE | EXT_POROSITY)) != 0u && (TRT || Q == 19);
 }
 // Uniform row bases (readfirstlane) for stream_collide: the forced D3Q19 16-bit kernels
-// drop from 96-99 to 58-66 VGPRs and gain 2-20 %; D3Q27 (27 slot bases in scalar
-// registers) and the unforced kernels lost up to 30 %, so they keep the plain bases.
-constexpr bool uniform_rows(int Q, uint32_t ST, uint32_t EXT) {
-	return Q == 19 && ST != ST_FP32 && (EXT & (EXT_VOLUME_FORCE | EXT_POROSITY)) != 0u;
+// drop from 96-99 to 58-66 VGPRs and gain 2-20 %, the unforced D3Q19 FP32 TRT kernels
+// 2-4 %; D3Q27 (27 slot bases in scalar registers) and the unforced 16-bit kernels lost up
+// to 30 %, the forced FP32 ones up to 1 %, so they keep the plain bases.
+template<bool TRT> constexpr bool uniform_rows(int Q, uint32_t ST, uint32_t EXT) {
+	const bool forced = (EXT & (EXT_VOLUME_FORCE | EXT_POROSITY)) != 0u;
+	if(Q != 19) return false;
+	return ST == ST_FP32 ? TRT && !forced : forced;
 }
 template<int Q, bool TRT>
 __device__ __forceinline__ void collide_pairs(float w, bool vf, bool isE, const Equilibrium<Q>& eq, const Forcing<Q>& fo, float* fhn) {
@@ -651,16 +654,16 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 		// every load is issued before the first decode: left alone, the scheduler may
 		// convert each 16-bit value right after its load and wait for it, one at a time
 		E r[Q];
-		load_raw<Q, ST, uniform_rows(Q, ST, EXT)>(fi, a.N, s, a.odd, r);
+		load_raw<Q, ST, uniform_rows<TRT>(Q, ST, EXT)>(fi, a.N, s, a.odd, r);
 		__builtin_amdgcn_sched_barrier(0);
 #pragma unroll
 		for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
 	} else {
-		load_f<Q, ST, uniform_rows(Q, ST, EXT)>(fi, a.N, s, a.odd, f);
+		load_f<Q, ST, uniform_rows<TRT>(Q, ST, EXT)>(fi, a.N, s, a.odd, f);
 	}
 	if((a.ext & EXT_MOVING_BOUNDARIES) && bo == TYPE_MS) moving_boundaries<Q>(a, s, f);
 	collide_cell<Q, TRT, EXT, RECORD>(a, n, bo, f, moments);
-	store_f<Q, ST, uniform_rows(Q, ST, EXT)>(fi, a.N, s, a.odd, f);
+	store_f<Q, ST, uniform_rows<TRT>(Q, ST, EXT)>(fi, a.N, s, a.odd, f);
 }
 // XCD: the kernel may run on the 1-D XCD-mapped grid (a.xchunks > 0); false compiles the
 // mapping out.
