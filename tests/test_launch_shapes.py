@@ -1,0 +1,85 @@
+"""Host-side launch logic of the HIP library (fluidx3d_amd/csrc/lbm_launch.hpp), compiled
+with hipcc into a small host program and run on the CPU: the residency cap reserves LDS in
+whole granules of 1/128 of the CU's LDS so that exactly the intended number of blocks fits
+(a 256-byte rounding let 54 528-byte blocks through, of which only 2, not 3, fit per CU);
+block shapes and the XCD-mapped grid cover every cell of a row."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+PROGRAM = r'''
+#include "lbm_launch.hpp"
+#include <cstdio>
+int main() {
+  using namespace fx;
+  for (unsigned bt : {64u, 128u, 256u}) for (unsigned w : {2u, 3u, 4u, 5u, 6u, 7u, 8u})
+    printf("cap %u %u %zu\n", bt, w, occupancy_lds(w, bt, 163840, 65536));
+  for (unsigned nx : {1u, 20u, 64u, 150u, 256u, 300u, 384u, 520u, 1030u, 1536u}) {
+    LaunchShape s = lattice_shape(nx, 7, 5), m = stream_shape(nx, 7, 5, true);
+    printf("shape %u %u %u %u %u %u %u\n", nx, s.block.x, s.grid.x, s.grid.y, s.grid.z, m.grid.x, m.xchunks);
+  }
+  printf("tune %u %u %u %u %u\n", tune_waves_cap(19, ST_FP32, 0), tune_waves_cap(27, ST_FP32, 0), tune_waves_cap(19, ST_FP16S, 0),
+         tune_waves_cap(19, ST_FP32, EXT_EQUILIBRIUM_BOUNDARIES), tune_waves_cap(19, ST_FP32, EXT_ALL));
+  return 0;
+}
+'''
+
+
+@pytest.fixture(scope="module")
+def launch_facts(tmp_path_factory):
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not available")
+    d = tmp_path_factory.mktemp("launch")
+    src, exe = d / "launch.hip", d / "launch"
+    src.write_text(PROGRAM)
+    subprocess.run([HIPCC, "--offload-arch=gfx950", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "fluidx3d_amd", "csrc"),
+                    "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True, capture_output=True, timeout=300)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True, timeout=60).stdout
+    caps, shapes, tune = {}, {}, None
+    for line in out.splitlines():
+        p = line.split()
+        if p[0] == "cap":
+            caps[(int(p[1]), int(p[2]))] = int(p[3])
+        elif p[0] == "shape":
+            shapes[int(p[1])] = tuple(int(v) for v in p[2:])
+        elif p[0] == "tune":
+            tune = tuple(int(v) for v in p[1:])
+    return caps, shapes, tune
+
+
+def test_residency_cap_reserves_whole_granules(launch_facts):
+    caps, _, _ = launch_facts
+    lds_per_cu, granule = 163840, 163840 // 128
+    assert caps[(256, 3)] == 53760  # 3 blocks of 4 waves: 3 waves/SIMD
+    for (threads, waves), lds in caps.items():
+        if lds == 0:
+            continue
+        blocks = waves * 4 // (threads // 64)
+        assert lds % granule == 0 and lds <= 65536
+        assert lds_per_cu // lds == blocks, (threads, waves, lds)
+    # 64-thread blocks at 5 waves/SIMD would need 20 blocks, which no whole number of
+    # granules gives exactly (7680 B: 21 blocks): no reservation rather than a wrong one
+    assert caps[(64, 5)] == 0
+
+
+def test_block_shapes_cover_the_rows(launch_facts):
+    _, shapes, _ = launch_facts
+    Ny, Nz = 7, 5
+    for nx, (bx, gx, gy, gz, mx, chunks) in shapes.items():
+        assert bx in (64, 128, 256) and gx * bx >= nx and (gx - 1) * bx < nx and (gy, gz) == (Ny, Nz)
+        assert bx == 64 or gx * bx - nx < 64  # less than one idle wave per row
+        if gx > 1:  # XCD-mapped 1-D grid: groups of 8 rows, every chunk of every row
+            assert chunks == gx and mx == ((Ny * Nz + 7) // 8) * 8 * gx
+        else:
+            assert chunks == 0
+
+
+def test_tuning_table(launch_facts):
+    _, _, tune = launch_facts
+    # plain FP32 D3Q19/D3Q27 at 3 waves/SIMD, FP16 uncapped, EQUILIBRIUM_BOUNDARIES at 4, generic uncapped
+    assert tune == (3, 3, 0, 4, 0)
