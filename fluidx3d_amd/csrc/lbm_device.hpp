@@ -80,13 +80,20 @@ template<uint32_t ST> struct Storage;
 // Measured at 256^3: FP32 D3Q19 non-temporal against cached at the unshifted slots +11 %,
 // cached x-shifted stores +4.7 %; D3Q27 also keeps cached x-shifted loads (+3 %); FP16S
 // +9 % from non-temporal loads and stores together at the unshifted slots (either alone
-// gains nothing).
+// gains nothing).  This holds where every row starts on a 128-byte line (Nx times the
+// element size a multiple of 128 bytes).  Otherwise the first and last line of a row are
+// shared with the neighbouring rows, and non-temporal accesses fetch and write such lines
+// in pieces: there the kernels take cached accesses at every slot (raw_cached,
+// store_cached), measured +21 to +37 % (FP32) and +18 to +61 % (FP16S) at 200^3, 300^3 and
+// 1000x250x250, while non-temporal accesses stay 16 % faster on aligned rows (256^3).
 template<> struct Storage<ST_FP32> {
 	typedef float E;
 	__device__ static __forceinline__ float raw(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
 	__device__ static __forceinline__ float decode(float x) { return x; }
 	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return raw(p, o); }
 	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { __builtin_nontemporal_store(x, elem(p, o)); }
+	__device__ static __forceinline__ float raw_cached(const E* p, uint32_t o) { return *elem(p, o); }
+	__device__ static __forceinline__ void store_cached(E* p, uint32_t o, float x) { *elem(p, o) = x; }
 	template<int Q> __device__ static __forceinline__ float raw_shifted(const E* p, uint32_t o) { return Q == 27 ? *elem(p, o) : raw(p, o); }
 	template<int Q> __device__ static __forceinline__ void store_shifted(E* p, uint32_t o, float x) { *elem(p, o) = x; }
 };
@@ -99,6 +106,8 @@ template<> struct Storage<ST_FP16S> {
 	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
 	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(raw(p, o)); }
 	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { __builtin_nontemporal_store(encode(x), elem(p, o)); }
+	__device__ static __forceinline__ uint16_t raw_cached(const E* p, uint32_t o) { return *elem(p, o); }
+	__device__ static __forceinline__ void store_cached(E* p, uint32_t o, float x) { *elem(p, o) = encode(x); }
 	template<int Q> __device__ static __forceinline__ uint16_t raw_shifted(const E* p, uint32_t o) { return *elem(p, o); }
 	template<int Q> __device__ static __forceinline__ void store_shifted(E* p, uint32_t o, float x) { *elem(p, o) = encode(x); }
 };
@@ -135,6 +144,8 @@ template<> struct Storage<ST_FP16C> {
 	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
 	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(raw(p, o)); }
 	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { __builtin_nontemporal_store(encode(x), elem(p, o)); }
+	__device__ static __forceinline__ uint16_t raw_cached(const E* p, uint32_t o) { return *elem(p, o); }
+	__device__ static __forceinline__ void store_cached(E* p, uint32_t o, float x) { *elem(p, o) = encode(x); }
 	template<int Q> __device__ static __forceinline__ uint16_t raw_shifted(const E* p, uint32_t o) { return *elem(p, o); }
 	template<int Q> __device__ static __forceinline__ void store_shifted(E* p, uint32_t o, float x) { *elem(p, o) = encode(x); }
 };
@@ -320,40 +331,45 @@ __device__ __forceinline__ Site make_site(uint32_t x, uint32_t y, uint32_t z, ui
 // and race-free.
 // U: the row offsets in s are the block's (wave-uniform); false where a lane has a row of
 // its own (the objective plane's one lane per (y, z)).
-template<int Q, uint32_t ST, bool U = false>
+// NT: non-temporal accesses at the unshifted slots (rows on 128-byte lines); false takes
+// cached accesses at every slot.
+template<int Q, uint32_t ST, bool U = false, bool NT = true>
 __device__ __forceinline__ void load_raw(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, typename Storage<ST>::E* r) {
 	typedef Storage<ST> S;
+	auto get = [](const typename S::E* p, uint32_t o) { return NT ? S::raw(p, o) : S::raw_cached(p, o); };
 	const uint64_t n0 = s.R[4];
-	r[0] = S::raw(row_base<U>(fi + n0), s.x);
+	r[0] = get(row_base<U>(fi + n0), s.x);
 #pragma unroll
 	for(int i = 1; i < Q; i += 2) {
 		const uint64_t sa = odd ? (uint64_t)i : (uint64_t)(i + 1);
 		const uint64_t sb = odd ? (uint64_t)(i + 1) : (uint64_t)i;
-		r[i] = S::raw(row_base<U>(fi + sa * N + n0), s.x);
+		r[i] = get(row_base<U>(fi + sa * N + n0), s.x);
 		const typename Storage<ST>::E* pb = row_base<U>(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]));
-		r[i + 1] = VS<Q>::cx[i] != 0 ? S::template raw_shifted<Q>(pb, s.X[VS<Q>::cx[i] + 1]) : S::raw(pb, s.x);
+		if(VS<Q>::cx[i] == 0) r[i + 1] = get(pb, s.x);
+		else r[i + 1] = NT ? S::template raw_shifted<Q>(pb, s.X[VS<Q>::cx[i] + 1]) : S::raw_cached(pb, s.X[VS<Q>::cx[i] + 1]);
 	}
 }
-template<int Q, uint32_t ST, bool U = false>
+template<int Q, uint32_t ST, bool U = false, bool NT = true>
 __device__ __forceinline__ void load_f(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, float* f) {
 	typename Storage<ST>::E r[Q];
-	load_raw<Q, ST, U>(fi, N, s, odd, r);
+	load_raw<Q, ST, U, NT>(fi, N, s, odd, r);
 #pragma unroll
 	for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
 }
-template<int Q, uint32_t ST, bool U = false>
+template<int Q, uint32_t ST, bool U = false, bool NT = true>
 __device__ __forceinline__ void store_f(typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, const float* f) {
 	typedef Storage<ST> S;
+	auto put = [](typename S::E* p, uint32_t o, float x) { if(NT) S::store(p, o, x); else S::store_cached(p, o, x); };
 	const uint64_t n0 = s.R[4];
-	S::store(row_base<U>(fi + n0), s.x, f[0]);
+	put(row_base<U>(fi + n0), s.x, f[0]);
 #pragma unroll
 	for(int i = 1; i < Q; i += 2) {
 		const uint64_t sa = odd ? (uint64_t)i : (uint64_t)(i + 1);
 		const uint64_t sb = odd ? (uint64_t)(i + 1) : (uint64_t)i;
 		typename Storage<ST>::E* pb = row_base<U>(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]));
 		if(VS<Q>::cx[i] != 0) S::template store_shifted<Q>(pb, s.X[VS<Q>::cx[i] + 1], f[i]);
-		else S::store(pb, s.x, f[i]);
-		S::store(row_base<U>(fi + sa * N + n0), s.x, f[i + 1]);
+		else put(pb, s.x, f[i]);
+		put(row_base<U>(fi + sa * N + n0), s.x, f[i + 1]);
 	}
 }
 
@@ -635,7 +651,7 @@ template<bool XCD> __device__ __forceinline__ bool block_coords(const LbmArgs& a
 	return true;
 }
 // One lane per cell.
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD, bool XCD>
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD, bool XCD, bool NT>
 __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 	a.ext &= EXT;
 	uint32_t chunk, y, z;
@@ -654,30 +670,31 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 		// every load is issued before the first decode: left alone, the scheduler may
 		// convert each 16-bit value right after its load and wait for it, one at a time
 		E r[Q];
-		load_raw<Q, ST, uniform_rows<TRT>(Q, ST, EXT)>(fi, a.N, s, a.odd, r);
+		load_raw<Q, ST, uniform_rows<TRT>(Q, ST, EXT), NT>(fi, a.N, s, a.odd, r);
 		__builtin_amdgcn_sched_barrier(0);
 #pragma unroll
 		for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
 	} else {
-		load_f<Q, ST, uniform_rows<TRT>(Q, ST, EXT)>(fi, a.N, s, a.odd, f);
+		load_f<Q, ST, uniform_rows<TRT>(Q, ST, EXT), NT>(fi, a.N, s, a.odd, f);
 	}
 	if((a.ext & EXT_MOVING_BOUNDARIES) && bo == TYPE_MS) moving_boundaries<Q>(a, s, f);
 	collide_cell<Q, TRT, EXT, RECORD>(a, n, bo, f, moments);
-	store_f<Q, ST, uniform_rows<TRT>(Q, ST, EXT)>(fi, a.N, s, a.odd, f);
+	store_f<Q, ST, uniform_rows<TRT>(Q, ST, EXT), NT>(fi, a.N, s, a.odd, f);
 }
 // XCD: the kernel may run on the 1-D XCD-mapped grid (a.xchunks > 0); false compiles the
-// mapping out.
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false>
+// mapping out.  NT: non-temporal DDF accesses at the unshifted slots, for lattices whose
+// rows start on 128-byte lines (see Storage).
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false, bool NT = true>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
-	stream_collide_cell<Q, ST, TRT, EXT, false, XCD>(a, nullptr);
+	stream_collide_cell<Q, ST, TRT, EXT, false, XCD, NT>(a, nullptr);
 }
 // stream_collide that also writes the pre-collision (rho, jx, jy, jz) of every fluid cell to
 // moments[c*N+n], c = 0..3 (16 B/cell): what the adjoint step needs of the forward state.
 // EXT is the context's extension subset; MOVING_BOUNDARIES and SUBGRID are not
 // differentiated and compiled out.
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false>
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false, bool NT = true>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide_record(LbmArgs a, float* moments) {
-	stream_collide_cell<Q, ST, TRT, EXT & ~(EXT_MOVING_BOUNDARIES | EXT_SUBGRID), true, XCD>(a, moments);
+	stream_collide_cell<Q, ST, TRT, EXT & ~(EXT_MOVING_BOUNDARIES | EXT_SUBGRID), true, XCD, NT>(a, moments);
 }
 
 // update_fields (src/kernel.cpp:1794-1870)
@@ -842,7 +859,7 @@ struct AdjointArgs {
 	float obj_weight;        // objective weight of step t (0 outside the averaging window)
 };
 
-template<int Q, bool TRT, bool XCD = false>
+template<int Q, bool TRT, bool XCD = false, bool NT = true>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_step(AdjointArgs g) {
 	const LbmArgs& a = g.a;
 	uint32_t chunk, y, z;
@@ -856,7 +873,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_step(AdjointArgs g) {
 	if(bo == TYPE_S || (flagsn & TYPE_SU) == TYPE_G) return;
 	float* lambda = (float*)a.fi;
 	float lam[Q], mu[Q];
-	load_f<Q, ST_FP32>(lambda, a.N, s, a.odd, lam); // lam[k] = lambda[L_k]
+	load_f<Q, ST_FP32, false, NT>(lambda, a.N, s, a.odd, lam); // lam[k] = lambda[L_k]
 	mu[0] = lam[0];
 #pragma unroll
 	for(int i = 1; i < Q; i += 2) { mu[i] = lam[i + 1]; mu[i + 1] = lam[i]; } // mu[m] = lambda[S_m]
@@ -975,7 +992,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_step(AdjointArgs g) {
 	out[0] = nu[0];
 #pragma unroll
 	for(int i = 1; i < Q; i += 2) { out[i] = nu[i + 1]; out[i + 1] = nu[i]; }
-	store_f<Q, ST_FP32>(lambda, a.N, s, a.odd, out);
+	store_f<Q, ST_FP32, false, NT>(lambda, a.N, s, a.odd, out);
 }
 
 // Objective of the adjoint: x-momentum sum_i c_ix v_i of the populations loaded at step t,

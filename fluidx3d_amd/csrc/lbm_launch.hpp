@@ -9,7 +9,12 @@ struct LaunchShape {
 	dim3 grid, block;
 	size_t lds = 0;       // dynamic LDS reserved per block (occupancy limit), see occupancy_lds()
 	uint32_t xchunks = 0; // 1-D XCD-mapped grid with this many x chunks per row (see block_coords)
+	bool nt = true;       // non-temporal DDF accesses (rows start on 128-byte lines, rows_on_lines)
 };
+// Every row of every DDF slot starts on a 128-byte line when Nx times the element size is
+// a multiple of 128 bytes (the slot stride N times the element size then is one too, and
+// device buffers are allocated 256-byte aligned).
+inline bool rows_on_lines(uint32_t Nx, uint32_t elem_bytes) { return (uint64_t)Nx * elem_bytes % 128u == 0u; }
 // Block of bx consecutive x cells of one row (bx = 64, 128 or 256); grid over (x chunk,
 // y, z).  bx is the widest block that leaves less than one idle wave per row (64 when none
 // does).
@@ -40,13 +45,15 @@ inline LaunchShape stream_shape(uint32_t Nx, uint32_t Ny, uint32_t Nz, bool xcd_
 // per CU), for a kernel that uses no LDS itself.  0 when the cap would not bind exactly.
 // A block's LDS is allocated in granules of 1/128 of the CU's LDS (1280 B of 160 KiB on
 // MI355X; measured: 53760 B per 256-thread block leaves 3 blocks per CU resident, 54272 B
-// only 2), so the size is a whole number of granules.
+// only 2), so the size is a whole number of granules, at most the per-block limit (2 blocks
+// of 256 threads: 65280 B).
 inline size_t occupancy_lds(uint32_t waves_per_simd, uint32_t block_threads, size_t lds_per_cu, size_t lds_per_block_max) {
 	const uint32_t waves_per_block = (block_threads + 63u) / 64u;
 	const uint32_t blocks = (waves_per_simd * 4u) / waves_per_block;
 	const size_t granule = lds_per_cu / 128u;
 	if(blocks == 0u || granule == 0u) return 0;
-	const size_t lds = (lds_per_cu / blocks) / granule * granule;
+	const size_t share = lds_per_cu / blocks;
+	const size_t lds = (share < lds_per_block_max ? share : lds_per_block_max) / granule * granule;
 	if(lds == 0u || lds > lds_per_block_max || lds_per_cu / lds != blocks) return 0;
 	return lds;
 }
@@ -75,6 +82,7 @@ struct LbmKernels {
 	bool stream_collide_xcd_map;    // 1-D XCD-mapped grid for rows of several blocks
 	transfer_fn transfer_fi;        // halo transfer of the DDFs (k_transfer_fi)
 	lattice_fn update_moving_boundaries;
+	uint32_t stream_collide_waves_cap_large; // cap for DDF arrays above LARGE_DDF_BYTES on aligned rows
 };
 
 constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY |
@@ -103,6 +111,18 @@ constexpr uint32_t tune_waves_cap(int Q, uint32_t ST, uint32_t EXT) {
 constexpr bool tune_xcd_map(int Q, uint32_t ST, uint32_t EXT) {
 	return true;
 }
+// Above 4 GiB of DDFs on rows that start on lines, plain D3Q19 FP32 streams 2-4 % faster
+// at 2 waves/SIMD than at 3 (measured at 7.7, 10.2 and 15 GB; +1.7 % at 5.1 GB, -0.3 % at
+// 2.6 GB, -2.4 % at 1.3 GB); D3Q27 FP32 stays fastest at 3 (+3-6 %) at every size.
+constexpr uint64_t LARGE_DDF_BYTES = 4ull << 30;
+constexpr uint32_t tune_waves_cap_large(int Q, uint32_t ST, uint32_t EXT) {
+	return (Q == 19 && ST == ST_FP32 && EXT == 0u) ? 2u : tune_waves_cap(Q, ST, EXT);
+}
+// Residency cap of stream_collide for a lattice of ddf_bytes of DDFs (0 = none); nt as in
+// LaunchShape.
+inline uint32_t stream_waves_cap(const LbmKernels& k, uint64_t ddf_bytes, bool nt) {
+	return nt && ddf_bytes > LARGE_DDF_BYTES ? k.stream_collide_waves_cap_large : k.stream_collide_waves_cap;
+}
 
 // Kernel set of one (velocity set, storage, collision) variant; EXT = 0 instantiates the
 // extension-free kernels, EXT_ALL the ones that read the runtime extension mask.
@@ -111,7 +131,8 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static void stream_collide(const LbmArgs& a, const LaunchShape& s, hipStream_t st) {
 		LbmArgs b = a;
 		b.xchunks = s.xchunks;
-		k_stream_collide<Q, ST, TRT, EXT, xcd_map><<<s.grid, s.block, s.lds, st>>>(b);
+		if(s.nt) k_stream_collide<Q, ST, TRT, EXT, xcd_map, true><<<s.grid, s.block, s.lds, st>>>(b);
+		else k_stream_collide<Q, ST, TRT, EXT, xcd_map, false><<<s.grid, s.block, s.lds, st>>>(b);
 	}
 	static void update_fields(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_fields<Q, ST, (EXT == 0u ? 0u : EXT_ALL)><<<s.grid, s.block, 0, st>>>(a); }
 	static void update_force_field(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_force_field<Q, ST><<<s.grid, s.block, 0, st>>>(a); }
@@ -122,12 +143,14 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static void stream_collide_record(const LbmArgs& a, const LaunchShape& s, hipStream_t st, float* m) {
 		LbmArgs b = a;
 		b.xchunks = s.xchunks;
-		k_stream_collide_record<Q, ST, TRT, EXT, true><<<s.grid, s.block, 0, st>>>(b, m);
+		if(s.nt) k_stream_collide_record<Q, ST, TRT, EXT, true, true><<<s.grid, s.block, 0, st>>>(b, m);
+		else k_stream_collide_record<Q, ST, TRT, EXT, true, false><<<s.grid, s.block, 0, st>>>(b, m);
 	}
 	static void adjoint_step(const AdjointArgs& g, const LaunchShape& s, hipStream_t st) {
 		AdjointArgs h = g;
 		h.a.xchunks = s.xchunks;
-		k_adjoint_step<Q, TRT, true><<<s.grid, s.block, 0, st>>>(h);
+		if(s.nt) k_adjoint_step<Q, TRT, true, true><<<s.grid, s.block, 0, st>>>(h);
+		else k_adjoint_step<Q, TRT, true, false><<<s.grid, s.block, 0, st>>>(h);
 	}
 	static void objective_partial(const LbmArgs& a, uint32_t plane_x, hipStream_t st, double* p) {
 		const uint64_t cells = (uint64_t)a.Ny * a.Nz;
@@ -151,7 +174,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static const LbmKernels* table() {
 		static const LbmKernels k = {initialize, stream_collide, update_fields, update_force_field, mass_partial, waves_cap,
 		                             stream_collide_record, adjoint_step, objective_partial, xcd_map, transfer_fi,
-		                             update_moving_boundaries};
+		                             update_moving_boundaries, tune_waves_cap_large(Q, ST, EXT)};
 		return &k;
 	}
 };

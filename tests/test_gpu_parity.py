@@ -74,7 +74,7 @@ def compare_state(lbm, ref, oracle, storage, tag):
     assert bad.size == 0, f"{tag}: {bad.size} DDF values differ, first slot/cell {divmod(int(bad[0]), lbm.N)}: gpu {g[bad[0]]!r} oracle {o[bad[0]]!r}"
 
 
-GRIDS = {"20x12x10": (20, 12, 10), "150x6x5": (150, 6, 5), "520x3x3": (520, 3, 3)}
+GRIDS = {"20x12x10": (20, 12, 10), "150x6x5": (150, 6, 5), "520x3x3": (520, 3, 3), "64x5x4": (64, 5, 4), "512x4x3": (512, 4, 3)}
 
 
 @pytest.mark.parametrize("grid", GRIDS)
@@ -84,7 +84,9 @@ def test_stream_collide_matches_oracle(oracle, q, collision, storage, ext, grid)
     both parities, ragged x-rows: Nx=20 in one 64-lane block; Nx=150, which with two cells
     per lane leaves a block with a full wave, a partly live wave and dead second cells;
     Nx=520, rows of several blocks on the 1-D XCD-mapped grid, whose last group of 8 rows
-    (9 rows here) is mostly padding."""
+    (9 rows here) is mostly padding.  These rows do not start on 128-byte lines, so the
+    kernels take cached DDF accesses; Nx=64 (one block) and Nx=512 (two blocks, XCD-mapped
+    grid) start every row on a line and take the non-temporal ones."""
     Nx, Ny, Nz = GRIDS[grid]
     names = EXT_SETS[ext]
     rho, u, flags = scenes.random_scene(Nx, Ny, Nz, seed=q + len(names), equilibrium_faces="EQUILIBRIUM_BOUNDARIES" in names)

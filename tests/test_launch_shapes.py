@@ -2,7 +2,8 @@
 with hipcc into a small host program and run on the CPU: the residency cap reserves LDS in
 whole granules of 1/128 of the CU's LDS so that exactly the intended number of blocks fits
 (a 256-byte rounding let 54 528-byte blocks through, of which only 2, not 3, fit per CU);
-block shapes and the XCD-mapped grid cover every cell of a row."""
+block shapes and the XCD-mapped grid cover every cell of a row; non-temporal DDF access is
+chosen exactly for rows that start on 128-byte lines."""
 import os
 import shutil
 import subprocess
@@ -25,6 +26,13 @@ int main() {
   }
   printf("tune %u %u %u %u %u\n", tune_waves_cap(19, ST_FP32, 0), tune_waves_cap(27, ST_FP32, 0), tune_waves_cap(19, ST_FP16S, 0),
          tune_waves_cap(19, ST_FP32, EXT_EQUILIBRIUM_BOUNDARIES), tune_waves_cap(19, ST_FP32, EXT_ALL));
+  printf("large %u %u %u\n", tune_waves_cap_large(19, ST_FP32, 0), tune_waves_cap_large(27, ST_FP32, 0), tune_waves_cap_large(19, ST_FP16S, 0));
+  LbmKernels k{};
+  k.stream_collide_waves_cap = 3; k.stream_collide_waves_cap_large = 2;
+  printf("capsel %u %u %u %u\n", stream_waves_cap(k, LARGE_DDF_BYTES, true), stream_waves_cap(k, LARGE_DDF_BYTES + 1, true),
+         stream_waves_cap(k, LARGE_DDF_BYTES + 1, false), stream_waves_cap(k, 1, true));
+  for (unsigned nx : {20u, 32u, 64u, 150u, 200u, 256u, 300u, 320u, 384u, 520u})
+    printf("lines %u %d %d\n", nx, (int)rows_on_lines(nx, 4), (int)rows_on_lines(nx, 2));
   return 0;
 }
 '''
@@ -40,22 +48,27 @@ def launch_facts(tmp_path_factory):
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "fluidx3d_amd", "csrc"),
                     "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True, capture_output=True, timeout=300)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True, timeout=60).stdout
-    caps, shapes, tune = {}, {}, None
+    caps, shapes, tune, lines = {}, {}, None, {}
     for line in out.splitlines():
         p = line.split()
         if p[0] == "cap":
             caps[(int(p[1]), int(p[2]))] = int(p[3])
         elif p[0] == "shape":
             shapes[int(p[1])] = tuple(int(v) for v in p[2:])
-        elif p[0] == "tune":
-            tune = tuple(int(v) for v in p[1:])
-    return caps, shapes, tune
+        elif p[0] in ("tune", "large", "capsel"):
+            tune = (tune or {}) | {p[0]: tuple(int(v) for v in p[1:])}
+        elif p[0] == "lines":
+            lines[int(p[1])] = (bool(int(p[2])), bool(int(p[3])))
+    return caps, shapes, tune, lines
 
 
 def test_residency_cap_reserves_whole_granules(launch_facts):
-    caps, _, _ = launch_facts
+    caps, _, _, _ = launch_facts
     lds_per_cu, granule = 163840, 163840 // 128
     assert caps[(256, 3)] == 53760  # 3 blocks of 4 waves: 3 waves/SIMD
+    # 2 blocks of 4 waves: the exact share (80 KiB) exceeds the 64 KiB per-block limit, the
+    # largest whole number of granules below it still leaves room for only 2 blocks
+    assert caps[(256, 2)] == 65280
     for (threads, waves), lds in caps.items():
         if lds == 0:
             continue
@@ -68,7 +81,7 @@ def test_residency_cap_reserves_whole_granules(launch_facts):
 
 
 def test_block_shapes_cover_the_rows(launch_facts):
-    _, shapes, _ = launch_facts
+    _, shapes, _, _ = launch_facts
     Ny, Nz = 7, 5
     for nx, (bx, gx, gy, gz, mx, chunks) in shapes.items():
         assert bx in (64, 128, 256) and gx * bx >= nx and (gx - 1) * bx < nx and (gy, gz) == (Ny, Nz)
@@ -80,6 +93,17 @@ def test_block_shapes_cover_the_rows(launch_facts):
 
 
 def test_tuning_table(launch_facts):
-    _, _, tune = launch_facts
+    _, _, tune, _ = launch_facts
     # plain FP32 D3Q19/D3Q27 at 3 waves/SIMD, FP16 uncapped, EQUILIBRIUM_BOUNDARIES at 4, generic uncapped
-    assert tune == (3, 3, 0, 4, 0)
+    assert tune["tune"] == (3, 3, 0, 4, 0)
+    # above 4 GiB of DDFs plain D3Q19 FP32 drops to 2 waves/SIMD, the others keep their cap
+    assert tune["large"] == (2, 3, 0)
+    # the large-lattice cap applies strictly above the limit and only with non-temporal access
+    assert tune["capsel"] == (3, 2, 3, 3)
+
+
+def test_access_policy_follows_row_alignment(launch_facts):
+    _, _, _, lines = launch_facts
+    for nx, (fp32, fp16) in lines.items():
+        assert fp32 == (nx * 4 % 128 == 0) and fp16 == (nx * 2 % 128 == 0), nx
+    assert lines[256] == (True, True) and lines[32] == (True, False) and lines[200] == (False, False)
