@@ -411,11 +411,11 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 	ctx->shape = fx::lattice_shape(cfg->Nx, cfg->Ny, cfg->Nz);
 	ctx->shape_sc = fx::stream_shape(cfg->Nx, cfg->Ny, cfg->Nz, ctx->k->stream_collide_xcd_map);
 	ctx->shape_adj = fx::adjoint_shape(cfg->Nx, cfg->Ny, cfg->Nz);
-	// non-temporal DDF accesses only where every row starts on a 128-byte line (the
-	// adjoint's FP32 lambda rows are then aligned too)
+	// rows that miss the 128-byte lines: cell-linear grid (the adjoint's FP32 lambda rows
+	// are aligned whenever the DDF rows are)
 	const uint32_t eb = cfg->storage == FX_FP32 ? 4u : 2u;
-	ctx->shape_sc.nt = ctx->shape_adj.nt = fx::rows_on_lines(cfg->Nx, eb);
-	const uint32_t cap = fx::stream_waves_cap(*ctx->k, N * cfg->velocity_set * eb, ctx->shape_sc.nt);
+	if(!fx::rows_on_lines(cfg->Nx, eb)) ctx->shape_sc = ctx->shape_adj = fx::linear_shape(N);
+	const uint32_t cap = fx::stream_waves_cap(*ctx->k, N * cfg->velocity_set * eb, ctx->shape_sc.lin);
 	hipError_t e = hipSetDevice(device);
 	if(e == hipSuccess) {
 		hipDeviceProp_t prop;

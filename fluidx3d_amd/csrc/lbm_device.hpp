@@ -80,12 +80,12 @@ template<uint32_t ST> struct Storage;
 // Measured at 256^3: FP32 D3Q19 non-temporal against cached at the unshifted slots +11 %,
 // cached x-shifted stores +4.7 %; D3Q27 also keeps cached x-shifted loads (+3 %); FP16S
 // +9 % from non-temporal loads and stores together at the unshifted slots (either alone
-// gains nothing).  This holds where every row starts on a 128-byte line (Nx times the
-// element size a multiple of 128 bytes).  Otherwise the first and last line of a row are
-// shared with the neighbouring rows, and non-temporal accesses fetch and write such lines
-// in pieces: there the kernels take cached accesses at every slot (raw_cached,
-// store_cached), measured +21 to +37 % (FP32) and +18 to +61 % (FP16S) at 200^3, 300^3 and
-// 1000x250x250, while non-temporal accesses stay 16 % faster on aligned rows (256^3).
+// gains nothing).  With rows that do not start on 128-byte lines (Nx times the element
+// size not a multiple of 128 bytes) the kernels run on the cell-linear grid (see
+// cell_coords): every wave then covers 64 consecutive cells, so its unshifted slots are
+// whole lines again and stay non-temporal, while the slots at shifted positions (offsets
+// of 1, Nx and Nx*Ny cells) straddle lines that neighbouring waves share and take cached
+// accesses (raw_cached, store_cached).
 template<> struct Storage<ST_FP32> {
 	typedef float E;
 	__device__ static __forceinline__ float raw(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
@@ -331,45 +331,46 @@ __device__ __forceinline__ Site make_site(uint32_t x, uint32_t y, uint32_t z, ui
 // and race-free.
 // U: the row offsets in s are the block's (wave-uniform); false where a lane has a row of
 // its own (the objective plane's one lane per (y, z)).
-// NT: non-temporal accesses at the unshifted slots (rows on 128-byte lines); false takes
-// cached accesses at every slot.
-template<int Q, uint32_t ST, bool U = false, bool NT = true>
+// SC: cached accesses at every shifted slot (the cell-linear grid); false: non-temporal
+// at the slots shifted in y/z only, the format's rule at the x-shifted ones.
+template<int Q, uint32_t ST, bool U = false, bool SC = false>
 __device__ __forceinline__ void load_raw(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, typename Storage<ST>::E* r) {
 	typedef Storage<ST> S;
-	auto get = [](const typename S::E* p, uint32_t o) { return NT ? S::raw(p, o) : S::raw_cached(p, o); };
 	const uint64_t n0 = s.R[4];
-	r[0] = get(row_base<U>(fi + n0), s.x);
+	r[0] = S::raw(row_base<U>(fi + n0), s.x);
 #pragma unroll
 	for(int i = 1; i < Q; i += 2) {
 		const uint64_t sa = odd ? (uint64_t)i : (uint64_t)(i + 1);
 		const uint64_t sb = odd ? (uint64_t)(i + 1) : (uint64_t)i;
-		r[i] = get(row_base<U>(fi + sa * N + n0), s.x);
+		r[i] = S::raw(row_base<U>(fi + sa * N + n0), s.x);
 		const typename Storage<ST>::E* pb = row_base<U>(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]));
-		if(VS<Q>::cx[i] == 0) r[i + 1] = get(pb, s.x);
-		else r[i + 1] = NT ? S::template raw_shifted<Q>(pb, s.X[VS<Q>::cx[i] + 1]) : S::raw_cached(pb, s.X[VS<Q>::cx[i] + 1]);
+		const uint32_t xb = s.X[VS<Q>::cx[i] + 1];
+		if(SC) r[i + 1] = S::raw_cached(pb, xb);
+		else r[i + 1] = VS<Q>::cx[i] != 0 ? S::template raw_shifted<Q>(pb, xb) : S::raw(pb, xb);
 	}
 }
-template<int Q, uint32_t ST, bool U = false, bool NT = true>
+template<int Q, uint32_t ST, bool U = false, bool SC = false>
 __device__ __forceinline__ void load_f(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, float* f) {
 	typename Storage<ST>::E r[Q];
-	load_raw<Q, ST, U, NT>(fi, N, s, odd, r);
+	load_raw<Q, ST, U, SC>(fi, N, s, odd, r);
 #pragma unroll
 	for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
 }
-template<int Q, uint32_t ST, bool U = false, bool NT = true>
+template<int Q, uint32_t ST, bool U = false, bool SC = false>
 __device__ __forceinline__ void store_f(typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, const float* f) {
 	typedef Storage<ST> S;
-	auto put = [](typename S::E* p, uint32_t o, float x) { if(NT) S::store(p, o, x); else S::store_cached(p, o, x); };
 	const uint64_t n0 = s.R[4];
-	put(row_base<U>(fi + n0), s.x, f[0]);
+	S::store(row_base<U>(fi + n0), s.x, f[0]);
 #pragma unroll
 	for(int i = 1; i < Q; i += 2) {
 		const uint64_t sa = odd ? (uint64_t)i : (uint64_t)(i + 1);
 		const uint64_t sb = odd ? (uint64_t)(i + 1) : (uint64_t)i;
 		typename Storage<ST>::E* pb = row_base<U>(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]));
-		if(VS<Q>::cx[i] != 0) S::template store_shifted<Q>(pb, s.X[VS<Q>::cx[i] + 1], f[i]);
-		else put(pb, s.x, f[i]);
-		put(row_base<U>(fi + sa * N + n0), s.x, f[i + 1]);
+		const uint32_t xb = s.X[VS<Q>::cx[i] + 1];
+		if(VS<Q>::cx[i] != 0) S::template store_shifted<Q>(pb, xb, f[i]);
+		else if(SC) S::store_cached(pb, xb, f[i]);
+		else S::store(pb, xb, f[i]);
+		S::store(row_base<U>(fi + sa * N + n0), s.x, f[i + 1]);
 	}
 }
 
@@ -650,8 +651,34 @@ template<bool XCD> __device__ __forceinline__ bool block_coords(const LbmArgs& a
 	y = row - z * a.Ny;
 	return true;
 }
+// Cell of this lane.  LIN: the cell-linear grid, block b covering cells b*blockDim.x ..
+// (b+1)*blockDim.x - 1 of the whole lattice, rows and planes crossed wherever they end
+// (every wave starts on a multiple of 64 cells, so its unshifted slot accesses are whole
+// 128-byte lines whatever Nx is); otherwise the row grid of block_coords.  False for lanes
+// past the lattice.
+template<bool XCD, bool LIN> __device__ __forceinline__ bool cell_coords(const LbmArgs& a, uint32_t& x, uint32_t& y, uint32_t& z) {
+	if constexpr(LIN) {
+		const uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x; // first cell of the block
+		if(c0 + threadIdx.x >= a.N) return false;
+		const uint64_t r0 = c0 / a.Nx; // its row y + z*Ny
+		const uint32_t z0 = (uint32_t)(r0 / a.Ny);
+		x = (uint32_t)(c0 - r0 * a.Nx) + threadIdx.x;
+		y = (uint32_t)(r0 - (uint64_t)z0 * a.Ny);
+		z = z0;
+		while(x >= a.Nx) { // rows crossed within the block (at most blockDim.x/Nx + 1)
+			x -= a.Nx;
+			if(++y == a.Ny) { y = 0u; z++; }
+		}
+		return true;
+	} else {
+		uint32_t chunk;
+		if(!block_coords<XCD>(a, chunk, y, z)) return false;
+		x = chunk * blockDim.x + threadIdx.x;
+		return x < a.Nx;
+	}
+}
 // One lane per cell.
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD, bool XCD, bool NT>
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD, bool XCD, bool LIN>
 __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 	a.ext &= EXT;
 	// kernels_for selects the VOLUME_FORCE subset only when exactly that extension is on, so
@@ -659,10 +686,9 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 	// Measured at 256^3: D3Q27 FP16S/FP16C +9 to +18 %; D3Q19 16-bit -7 to -14 % (58-64 ->
 	// 76-79 VGPRs, 8 -> 6 waves/SIMD), FP32 -5 to +1.5 %, so only D3Q27 16-bit takes it.
 	if constexpr(Q == 27 && ST != ST_FP32 && EXT == EXT_VOLUME_FORCE) a.ext = EXT_VOLUME_FORCE;
-	uint32_t chunk, y, z;
-	if(!block_coords<XCD>(a, chunk, y, z)) return;
-	const uint32_t x = chunk * blockDim.x + threadIdx.x;
-	if(x >= a.Nx) return;
+	uint32_t x, y, z;
+	if(!cell_coords<XCD, LIN>(a, x, y, z)) return;
+	constexpr bool U = !LIN && uniform_rows<TRT>(Q, ST, EXT); // rows are per lane on the cell-linear grid
 	const Site s = make_site(x, y, z, a.Nx, a.Ny, a.Nz);
 	const uint64_t n = s.n();
 	const uint32_t flagsn = a.flags[n];
@@ -675,31 +701,31 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 		// every load is issued before the first decode: left alone, the scheduler may
 		// convert each 16-bit value right after its load and wait for it, one at a time
 		E r[Q];
-		load_raw<Q, ST, uniform_rows<TRT>(Q, ST, EXT), NT>(fi, a.N, s, a.odd, r);
+		load_raw<Q, ST, U, LIN>(fi, a.N, s, a.odd, r);
 		__builtin_amdgcn_sched_barrier(0);
 #pragma unroll
 		for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
 	} else {
-		load_f<Q, ST, uniform_rows<TRT>(Q, ST, EXT), NT>(fi, a.N, s, a.odd, f);
+		load_f<Q, ST, U, LIN>(fi, a.N, s, a.odd, f);
 	}
 	if((a.ext & EXT_MOVING_BOUNDARIES) && bo == TYPE_MS) moving_boundaries<Q>(a, s, f);
 	collide_cell<Q, TRT, EXT, RECORD>(a, n, bo, f, moments);
-	store_f<Q, ST, uniform_rows<TRT>(Q, ST, EXT), NT>(fi, a.N, s, a.odd, f);
+	store_f<Q, ST, U, LIN>(fi, a.N, s, a.odd, f);
 }
 // XCD: the kernel may run on the 1-D XCD-mapped grid (a.xchunks > 0); false compiles the
-// mapping out.  NT: non-temporal DDF accesses at the unshifted slots, for lattices whose
-// rows start on 128-byte lines (see Storage).
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false, bool NT = true>
+// mapping out.  LIN: the cell-linear grid (cell_coords), for lattices whose rows do not
+// start on 128-byte lines.
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false, bool LIN = false>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
-	stream_collide_cell<Q, ST, TRT, EXT, false, XCD, NT>(a, nullptr);
+	stream_collide_cell<Q, ST, TRT, EXT, false, XCD, LIN>(a, nullptr);
 }
 // stream_collide that also writes the pre-collision (rho, jx, jy, jz) of every fluid cell to
 // moments[c*N+n], c = 0..3 (16 B/cell): what the adjoint step needs of the forward state.
 // EXT is the context's extension subset; MOVING_BOUNDARIES and SUBGRID are not
 // differentiated and compiled out.
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false, bool NT = true>
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false, bool LIN = false>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide_record(LbmArgs a, float* moments) {
-	stream_collide_cell<Q, ST, TRT, EXT & ~(EXT_MOVING_BOUNDARIES | EXT_SUBGRID), true, XCD, NT>(a, moments);
+	stream_collide_cell<Q, ST, TRT, EXT & ~(EXT_MOVING_BOUNDARIES | EXT_SUBGRID), true, XCD, LIN>(a, moments);
 }
 
 // update_fields (src/kernel.cpp:1794-1870)
@@ -864,13 +890,11 @@ struct AdjointArgs {
 	float obj_weight;        // objective weight of step t (0 outside the averaging window)
 };
 
-template<int Q, bool TRT, bool XCD = false, bool NT = true>
+template<int Q, bool TRT, bool XCD = false, bool LIN = false>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_step(AdjointArgs g) {
 	const LbmArgs& a = g.a;
-	uint32_t chunk, y, z;
-	if(!block_coords<XCD>(a, chunk, y, z)) return;
-	const uint32_t x = chunk * blockDim.x + threadIdx.x;
-	if(x >= a.Nx) return;
+	uint32_t x, y, z;
+	if(!cell_coords<XCD, LIN>(a, x, y, z)) return;
 	const Site s = make_site(x, y, z, a.Nx, a.Ny, a.Nz);
 	const uint64_t n = s.n();
 	const uint32_t flagsn = a.flags[n];
@@ -878,7 +902,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_step(AdjointArgs g) {
 	if(bo == TYPE_S || (flagsn & TYPE_SU) == TYPE_G) return;
 	float* lambda = (float*)a.fi;
 	float lam[Q], mu[Q];
-	load_f<Q, ST_FP32, false, NT>(lambda, a.N, s, a.odd, lam); // lam[k] = lambda[L_k]
+	load_f<Q, ST_FP32, false, LIN>(lambda, a.N, s, a.odd, lam); // lam[k] = lambda[L_k]
 	mu[0] = lam[0];
 #pragma unroll
 	for(int i = 1; i < Q; i += 2) { mu[i] = lam[i + 1]; mu[i + 1] = lam[i]; } // mu[m] = lambda[S_m]
@@ -997,7 +1021,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_step(AdjointArgs g) {
 	out[0] = nu[0];
 #pragma unroll
 	for(int i = 1; i < Q; i += 2) { out[i] = nu[i + 1]; out[i + 1] = nu[i]; }
-	store_f<Q, ST_FP32, false, NT>(lambda, a.N, s, a.odd, out);
+	store_f<Q, ST_FP32, false, LIN>(lambda, a.N, s, a.odd, out);
 }
 
 // Objective of the adjoint: x-momentum sum_i c_ix v_i of the populations loaded at step t,

@@ -85,8 +85,9 @@ def test_stream_collide_matches_oracle(oracle, q, collision, storage, ext, grid)
     per lane leaves a block with a full wave, a partly live wave and dead second cells;
     Nx=520, rows of several blocks on the 1-D XCD-mapped grid, whose last group of 8 rows
     (9 rows here) is mostly padding.  These rows do not start on 128-byte lines, so the
-    kernels take cached DDF accesses; Nx=64 (one block) and Nx=512 (two blocks, XCD-mapped
-    grid) start every row on a line and take the non-temporal ones."""
+    kernels run on the cell-linear grid, whose blocks cross rows and planes (Nx=20: 12.8
+    rows per block); Nx=64 (one block per row) and Nx=512 (two blocks, XCD-mapped grid)
+    start every row on a line and keep the row grid."""
     Nx, Ny, Nz = GRIDS[grid]
     names = EXT_SETS[ext]
     rho, u, flags = scenes.random_scene(Nx, Ny, Nz, seed=q + len(names), equilibrium_faces="EQUILIBRIUM_BOUNDARIES" in names)

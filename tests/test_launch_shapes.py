@@ -2,8 +2,8 @@
 with hipcc into a small host program and run on the CPU: the residency cap reserves LDS in
 whole granules of 1/128 of the CU's LDS so that exactly the intended number of blocks fits
 (a 256-byte rounding let 54 528-byte blocks through, of which only 2, not 3, fit per CU);
-block shapes and the XCD-mapped grid cover every cell of a row; non-temporal DDF access is
-chosen exactly for rows that start on 128-byte lines."""
+block shapes and the XCD-mapped grid cover every cell of a row; the row grid is kept
+exactly for rows that start on 128-byte lines, the cell-linear grid covers the others."""
 import os
 import shutil
 import subprocess
@@ -29,8 +29,12 @@ int main() {
   printf("large %u %u %u\n", tune_waves_cap_large(19, ST_FP32, 0), tune_waves_cap_large(27, ST_FP32, 0), tune_waves_cap_large(19, ST_FP16S, 0));
   LbmKernels k{};
   k.stream_collide_waves_cap = 3; k.stream_collide_waves_cap_large = 2;
-  printf("capsel %u %u %u %u\n", stream_waves_cap(k, LARGE_DDF_BYTES, true), stream_waves_cap(k, LARGE_DDF_BYTES + 1, true),
-         stream_waves_cap(k, LARGE_DDF_BYTES + 1, false), stream_waves_cap(k, 1, true));
+  printf("capsel %u %u %u %u\n", stream_waves_cap(k, LARGE_DDF_BYTES, false), stream_waves_cap(k, LARGE_DDF_BYTES + 1, false),
+         stream_waves_cap(k, LARGE_DDF_BYTES + 1, true), stream_waves_cap(k, 1, false));
+  for (unsigned long long n : {1ull, 255ull, 256ull, 257ull, 8000000ull, 906000000ull}) {
+    LaunchShape l = linear_shape(n);
+    printf("linear %llu %u %u %u %u %d\n", n, l.grid.x, l.grid.y, l.grid.z, l.block.x, (int)l.lin);
+  }
   for (unsigned nx : {20u, 32u, 64u, 150u, 200u, 256u, 300u, 320u, 384u, 520u})
     printf("lines %u %d %d\n", nx, (int)rows_on_lines(nx, 4), (int)rows_on_lines(nx, 2));
   return 0;
@@ -59,6 +63,8 @@ def launch_facts(tmp_path_factory):
             tune = (tune or {}) | {p[0]: tuple(int(v) for v in p[1:])}
         elif p[0] == "lines":
             lines[int(p[1])] = (bool(int(p[2])), bool(int(p[3])))
+        elif p[0] == "linear":
+            lines[("linear", int(p[1]))] = tuple(int(v) for v in p[2:])
     return caps, shapes, tune, lines
 
 
@@ -98,12 +104,20 @@ def test_tuning_table(launch_facts):
     assert tune["tune"] == (3, 3, 0, 4, 0)
     # above 4 GiB of DDFs plain D3Q19 FP32 drops to 2 waves/SIMD, the others keep their cap
     assert tune["large"] == (2, 3, 0)
-    # the large-lattice cap applies strictly above the limit and only with non-temporal access
+    # the large-lattice cap applies strictly above the limit and only on the row grid
     assert tune["capsel"] == (3, 2, 3, 3)
 
 
-def test_access_policy_follows_row_alignment(launch_facts):
+def test_grid_follows_row_alignment(launch_facts):
+    """Rows on 128-byte lines keep the row grid; the others take the cell-linear grid, whose
+    blocks of 256 consecutive cells cover the lattice exactly once."""
     _, _, _, lines = launch_facts
-    for nx, (fp32, fp16) in lines.items():
-        assert fp32 == (nx * 4 % 128 == 0) and fp16 == (nx * 2 % 128 == 0), nx
+    for key, v in lines.items():
+        if isinstance(key, tuple):
+            n = key[1]
+            gx, gy, gz, bx, lin = v
+            assert lin == 1 and bx == 256 and (gy, gz) == (1, 1) and gx * 256 >= n and (gx - 1) * 256 < n, key
+        else:
+            fp32, fp16 = v
+            assert fp32 == (key * 4 % 128 == 0) and fp16 == (key * 2 % 128 == 0), key
     assert lines[256] == (True, True) and lines[32] == (True, False) and lines[200] == (False, False)
