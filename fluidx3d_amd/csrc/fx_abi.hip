@@ -414,7 +414,10 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 	// rows that miss the 128-byte lines: cell-linear grid (the adjoint's FP32 lambda rows
 	// are aligned whenever the DDF rows are)
 	const uint32_t eb = cfg->storage == FX_FP32 ? 4u : 2u;
-	if(!fx::rows_on_lines(cfg->Nx, eb)) ctx->shape_sc = ctx->shape_adj = fx::linear_shape(N);
+	if(!fx::rows_on_lines(cfg->Nx, eb)) {
+		ctx->shape_sc = fx::linear_shape(N, ctx->k->stream_collide_linear_xcd);
+		ctx->shape_adj = fx::linear_shape(N, false);
+	}
 	const uint32_t cap = fx::stream_waves_cap(*ctx->k, N * cfg->velocity_set * eb, ctx->shape_sc.lin);
 	hipError_t e = hipSetDevice(device);
 	if(e == hipSuccess) {

@@ -651,14 +651,23 @@ template<bool XCD> __device__ __forceinline__ bool block_coords(const LbmArgs& a
 	y = row - z * a.Ny;
 	return true;
 }
-// Cell of this lane.  LIN: the cell-linear grid, block b covering cells b*blockDim.x ..
-// (b+1)*blockDim.x - 1 of the whole lattice, rows and planes crossed wherever they end
-// (every wave starts on a multiple of 64 cells, so its unshifted slot accesses are whole
-// 128-byte lines whatever Nx is); otherwise the row grid of block_coords.  False for lanes
-// past the lattice.
+// Segment (block of consecutive cells) of block L on the cell-linear grid.  The hardware
+// hands consecutive blocks to the 8 XCDs in turn; with xcd and a grid of whole groups of
+// 64 blocks, group g's block 8j + k takes segment 64g + 8k + j, so each XCD runs 8
+// consecutive segments and the cache lines that neighbouring segments share at the shifted
+// slots mostly stay in one L2.  A permutation of each group; the identity otherwise.
+__host__ __device__ inline uint64_t linear_segment(uint64_t L, uint64_t blocks, bool xcd) {
+	if(!xcd || (blocks & 63u) != 0u) return L;
+	return (L & ~(uint64_t)63u) | ((L & 7u) << 3) | ((L >> 3) & 7u);
+}
+// Cell of this lane.  LIN: the cell-linear grid, block L covering segment
+// linear_segment(L) of blockDim.x consecutive cells of the whole lattice, rows and planes
+// crossed wherever they end (every wave starts on a multiple of 64 cells, so its unshifted
+// slot accesses are whole 128-byte lines whatever Nx is); XCD there selects the XCD-aware
+// segment order.  Otherwise the row grid of block_coords.  False for lanes past the lattice.
 template<bool XCD, bool LIN> __device__ __forceinline__ bool cell_coords(const LbmArgs& a, uint32_t& x, uint32_t& y, uint32_t& z) {
 	if constexpr(LIN) {
-		const uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x; // first cell of the block
+		const uint64_t c0 = linear_segment(blockIdx.x, gridDim.x, XCD) * blockDim.x; // first cell of the block
 		if(c0 + threadIdx.x >= a.N) return false;
 		const uint64_t r0 = c0 / a.Nx; // its row y + z*Ny
 		const uint32_t z0 = (uint32_t)(r0 / a.Ny);

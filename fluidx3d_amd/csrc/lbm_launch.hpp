@@ -18,11 +18,14 @@ inline bool rows_on_lines(uint32_t Nx, uint32_t elem_bytes) { return (uint64_t)N
 // stream_collide and the adjoint kernels on lattices whose rows miss the lines: blocks of
 // 256 consecutive cells of the whole lattice.  Measured (plain D3Q19, against blocks over
 // rows with cached accesses): 200^3 FP32 +21 %, FP16S +24 %; 300^3 +16 %, +23 %;
-// 1000x250x250 +12 %, +3 %; 352x352x256 FP16S +19 %.
-inline LaunchShape linear_shape(uint64_t N) {
+// 1000x250x250 +12 %, +3 %; 352x352x256 FP16S +19 %.  xcd pads the grid to whole groups
+// of 64 blocks for the XCD-aware segment order (linear_segment).
+inline LaunchShape linear_shape(uint64_t N, bool xcd) {
 	LaunchShape s;
+	uint64_t blocks = (N + 255u) / 256u;
+	if(xcd) blocks = (blocks + 63u) / 64u * 64u;
 	s.block = dim3(256, 1, 1);
-	s.grid = dim3((unsigned)((N + 255u) / 256u), 1, 1);
+	s.grid = dim3((unsigned)blocks, 1, 1);
 	s.lin = true;
 	return s;
 }
@@ -94,6 +97,7 @@ struct LbmKernels {
 	transfer_fn transfer_fi;        // halo transfer of the DDFs (k_transfer_fi)
 	lattice_fn update_moving_boundaries;
 	uint32_t stream_collide_waves_cap_large; // cap for DDF arrays above LARGE_DDF_BYTES on the row grid
+	bool stream_collide_linear_xcd;          // XCD-aware segment order on the cell-linear grid
 };
 
 constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY |
@@ -126,6 +130,9 @@ constexpr bool tune_xcd_map(int Q, uint32_t ST, uint32_t EXT) {
 // at 2 waves/SIMD than at 3 (measured at 7.7, 10.2 and 15 GB; +1.7 % at 5.1 GB, -0.3 % at
 // 2.6 GB, -2.4 % at 1.3 GB); D3Q27 FP32 stays fastest at 3 (+3-6 %) at every size.
 constexpr uint64_t LARGE_DDF_BYTES = 4ull << 30;
+// On the cell-linear grid the XCD-aware segment order gains 3-7 % for FP16S (200^3 to
+// 1000x250x250, full occupancy) and -2 to +1 % for FP32 at 3 waves/SIMD.
+constexpr bool tune_linear_xcd(int Q, uint32_t ST, uint32_t EXT) { return ST != ST_FP32; }
 constexpr uint32_t tune_waves_cap_large(int Q, uint32_t ST, uint32_t EXT) {
 	return (Q == 19 && ST == ST_FP32 && EXT == 0u) ? 2u : tune_waves_cap(Q, ST, EXT);
 }
@@ -142,7 +149,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static void stream_collide(const LbmArgs& a, const LaunchShape& s, hipStream_t st) {
 		LbmArgs b = a;
 		b.xchunks = s.xchunks;
-		if(s.lin) k_stream_collide<Q, ST, TRT, EXT, false, true><<<s.grid, s.block, s.lds, st>>>(b);
+		if(s.lin) k_stream_collide<Q, ST, TRT, EXT, linear_xcd, true><<<s.grid, s.block, s.lds, st>>>(b);
 		else k_stream_collide<Q, ST, TRT, EXT, xcd_map, false><<<s.grid, s.block, s.lds, st>>>(b);
 	}
 	static void update_fields(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_fields<Q, ST, (EXT == 0u ? 0u : EXT_ALL)><<<s.grid, s.block, 0, st>>>(a); }
@@ -170,6 +177,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	// Launch tuning of stream_collide: residency cap in waves per SIMD (0 = none) and grid.
 	static constexpr uint32_t waves_cap = tune_waves_cap(Q, ST, EXT);
 	static constexpr bool xcd_map = tune_xcd_map(Q, ST, EXT);
+	static constexpr bool linear_xcd = tune_linear_xcd(Q, ST, EXT);
 	template<int AXIS> static void transfer_axis(void* fi, void* bp, void* bm, uint32_t Nx, uint32_t Ny, uint32_t Nz, bool insert, uint32_t odd, hipStream_t st) {
 		typedef typename Storage<ST>::E E;
 		const uint64_t A = AXIS == 0 ? (uint64_t)Ny * Nz : AXIS == 1 ? (uint64_t)Nx * Nz : (uint64_t)Nx * Ny;
@@ -185,7 +193,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static const LbmKernels* table() {
 		static const LbmKernels k = {initialize, stream_collide, update_fields, update_force_field, mass_partial, waves_cap,
 		                             stream_collide_record, adjoint_step, objective_partial, xcd_map, transfer_fi,
-		                             update_moving_boundaries, tune_waves_cap_large(Q, ST, EXT)};
+		                             update_moving_boundaries, tune_waves_cap_large(Q, ST, EXT), linear_xcd};
 		return &k;
 	}
 };

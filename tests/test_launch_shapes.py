@@ -16,6 +16,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 PROGRAM = r'''
 #include "lbm_launch.hpp"
 #include <cstdio>
+#include <vector>
 int main() {
   using namespace fx;
   for (unsigned bt : {64u, 128u, 256u}) for (unsigned w : {2u, 3u, 4u, 5u, 6u, 7u, 8u})
@@ -31,9 +32,24 @@ int main() {
   k.stream_collide_waves_cap = 3; k.stream_collide_waves_cap_large = 2;
   printf("capsel %u %u %u %u\n", stream_waves_cap(k, LARGE_DDF_BYTES, false), stream_waves_cap(k, LARGE_DDF_BYTES + 1, false),
          stream_waves_cap(k, LARGE_DDF_BYTES + 1, true), stream_waves_cap(k, 1, false));
-  for (unsigned long long n : {1ull, 255ull, 256ull, 257ull, 8000000ull, 906000000ull}) {
-    LaunchShape l = linear_shape(n);
-    printf("linear %llu %u %u %u %u %d\n", n, l.grid.x, l.grid.y, l.grid.z, l.block.x, (int)l.lin);
+  for (unsigned long long n : {1ull, 255ull, 256ull, 257ull, 8000000ull, 906000000ull}) for (int xcd = 0; xcd < 2; xcd++) {
+    LaunchShape l = linear_shape(n, xcd != 0);
+    printf("linear %llu %d %u %u %u %u %d\n", n, xcd, l.grid.x, l.grid.y, l.grid.z, l.block.x, (int)l.lin);
+  }
+  // segment order: every segment exactly once, each XCD (L mod 8) on runs of 8 consecutive
+  // segments, for padded grids; the identity for unpadded ones
+  for (unsigned long long blocks : {64ull, 128ull, 31296ull, 100ull}) {
+    std::vector<int> seen(blocks, 0);
+    unsigned long long runs_bad = 0, moved = 0;
+    for (unsigned long long L = 0; L < blocks; L++) {
+      const unsigned long long g = linear_segment(L, blocks, true);
+      if (g < blocks) seen[g]++;
+      moved += g != L;
+      if (blocks % 64 == 0 && linear_segment(L, blocks, true) / 8 != (L / 64) * 8 + L % 8) runs_bad++;
+    }
+    int once = 1;
+    for (int v : seen) once &= v == 1;
+    printf("segment %llu %d %llu %llu\n", blocks, once, runs_bad, moved);
   }
   for (unsigned nx : {20u, 32u, 64u, 150u, 200u, 256u, 300u, 320u, 384u, 520u})
     printf("lines %u %d %d\n", nx, (int)rows_on_lines(nx, 4), (int)rows_on_lines(nx, 2));
@@ -64,7 +80,9 @@ def launch_facts(tmp_path_factory):
         elif p[0] == "lines":
             lines[int(p[1])] = (bool(int(p[2])), bool(int(p[3])))
         elif p[0] == "linear":
-            lines[("linear", int(p[1]))] = tuple(int(v) for v in p[2:])
+            lines[("linear", int(p[1]), int(p[2]))] = tuple(int(v) for v in p[3:])
+        elif p[0] == "segment":
+            lines[("segment", int(p[1]))] = tuple(int(v) for v in p[2:])
     return caps, shapes, tune, lines
 
 
@@ -113,10 +131,19 @@ def test_grid_follows_row_alignment(launch_facts):
     blocks of 256 consecutive cells cover the lattice exactly once."""
     _, _, _, lines = launch_facts
     for key, v in lines.items():
-        if isinstance(key, tuple):
-            n = key[1]
+        if isinstance(key, tuple) and key[0] == "linear":
+            n, xcd = key[1], key[2]
             gx, gy, gz, bx, lin = v
-            assert lin == 1 and bx == 256 and (gy, gz) == (1, 1) and gx * 256 >= n and (gx - 1) * 256 < n, key
+            need = (n + 255) // 256
+            assert lin == 1 and bx == 256 and (gy, gz) == (1, 1), key
+            if xcd:  # padded to whole groups of 64 blocks for the XCD-aware segment order
+                assert gx % 64 == 0 and need <= gx < need + 64, key
+            else:
+                assert gx == need, key
+        elif isinstance(key, tuple) and key[0] == "segment":
+            once, runs_bad, moved = v
+            assert once == 1 and runs_bad == 0, key  # a permutation; XCD k runs segments 8k..8k+7 of each group
+            assert (moved == 0) == (key[1] % 64 != 0), key  # identity exactly for unpadded grids
         else:
             fp32, fp16 = v
             assert fp32 == (key * 4 % 128 == 0) and fp16 == (key * 2 % 128 == 0), key
