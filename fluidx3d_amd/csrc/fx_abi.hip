@@ -411,14 +411,14 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 	ctx->shape = fx::lattice_shape(cfg->Nx, cfg->Ny, cfg->Nz);
 	ctx->shape_sc = fx::stream_shape(cfg->Nx, cfg->Ny, cfg->Nz, ctx->k->stream_collide_xcd_map);
 	ctx->shape_adj = fx::adjoint_shape(cfg->Nx, cfg->Ny, cfg->Nz);
-	// rows that miss the 128-byte lines: cell-linear grid (the adjoint's FP32 lambda rows
-	// are aligned whenever the DDF rows are)
+	// cell-linear grid for rows that miss the 128-byte lines and for large FP32 lattices
+	// (the adjoint's FP32 lambda rows are aligned whenever the DDF rows are)
 	const uint32_t eb = cfg->storage == FX_FP32 ? 4u : 2u;
-	if(!fx::rows_on_lines(cfg->Nx, eb)) {
+	if(fx::use_linear_grid(*ctx->k, cfg->Nx, eb, N * cfg->velocity_set * eb)) {
 		ctx->shape_sc = fx::linear_shape(N, ctx->k->stream_collide_linear_xcd);
 		ctx->shape_adj = fx::linear_shape(N, false);
 	}
-	const uint32_t cap = fx::stream_waves_cap(*ctx->k, N * cfg->velocity_set * eb, ctx->shape_sc.lin);
+	const uint32_t cap = ctx->k->stream_collide_waves_cap;
 	hipError_t e = hipSetDevice(device);
 	if(e == hipSuccess) {
 		hipDeviceProp_t prop;

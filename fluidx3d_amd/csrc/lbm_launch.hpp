@@ -96,8 +96,8 @@ struct LbmKernels {
 	bool stream_collide_xcd_map;    // 1-D XCD-mapped grid for rows of several blocks
 	transfer_fn transfer_fi;        // halo transfer of the DDFs (k_transfer_fi)
 	lattice_fn update_moving_boundaries;
-	uint32_t stream_collide_waves_cap_large; // cap for DDF arrays above LARGE_DDF_BYTES on the row grid
-	bool stream_collide_linear_xcd;          // XCD-aware segment order on the cell-linear grid
+	bool stream_collide_linear_large; // cell-linear grid for DDF arrays above LARGE_DDF_BYTES
+	bool stream_collide_linear_xcd;   // XCD-aware segment order on the cell-linear grid
 };
 
 constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY |
@@ -126,20 +126,22 @@ constexpr uint32_t tune_waves_cap(int Q, uint32_t ST, uint32_t EXT) {
 constexpr bool tune_xcd_map(int Q, uint32_t ST, uint32_t EXT) {
 	return true;
 }
-// Above 4 GiB of DDFs on rows that start on lines, plain D3Q19 FP32 streams 2-4 % faster
-// at 2 waves/SIMD than at 3 (measured at 7.7, 10.2 and 15 GB; +1.7 % at 5.1 GB, -0.3 % at
-// 2.6 GB, -2.4 % at 1.3 GB); D3Q27 FP32 stays fastest at 3 (+3-6 %) at every size.
+// Above 4 GiB of DDFs the FP32 kernels stream faster on the cell-linear grid than on the
+// row grid, also where the rows start on lines (same residency cap).  Plain D3Q19: 384^3
+// +10 %, 256x512x512 +2 %, 512^3 +12 to +16 %, 768x512x512 +2.5 %, 1024x512x512 +11 %;
+// the D3Q19 EQUILIBRIUM_BOUNDARIES, VOLUME_FORCE, +FORCE_FIELD and porous kernels +12 to
+// +14 % at 512^3 and +4 to +5 % at 768x512x512; plain D3Q27 +0.4 to +1 %.  Below, the row
+// grid is ahead (256^3 +10 %, 512x256x256 +2 %).  The 16-bit formats keep the row grid at
+// every size (+1 to +7 %).
 constexpr uint64_t LARGE_DDF_BYTES = 4ull << 30;
+constexpr bool tune_linear_large(int Q, uint32_t ST, uint32_t EXT) { return ST == ST_FP32; }
 // On the cell-linear grid the XCD-aware segment order gains 3-7 % for FP16S (200^3 to
 // 1000x250x250, full occupancy) and -2 to +1 % for FP32 at 3 waves/SIMD.
 constexpr bool tune_linear_xcd(int Q, uint32_t ST, uint32_t EXT) { return ST != ST_FP32; }
-constexpr uint32_t tune_waves_cap_large(int Q, uint32_t ST, uint32_t EXT) {
-	return (Q == 19 && ST == ST_FP32 && EXT == 0u) ? 2u : tune_waves_cap(Q, ST, EXT);
-}
-// Residency cap of stream_collide for a lattice of ddf_bytes of DDFs (0 = none); lin as in
-// LaunchShape.
-inline uint32_t stream_waves_cap(const LbmKernels& k, uint64_t ddf_bytes, bool lin) {
-	return !lin && ddf_bytes > LARGE_DDF_BYTES ? k.stream_collide_waves_cap_large : k.stream_collide_waves_cap;
+// stream_collide (and the adjoint kernels) on the cell-linear grid: rows that miss the
+// 128-byte lines, or a large FP32 lattice.
+inline bool use_linear_grid(const LbmKernels& k, uint32_t Nx, uint32_t elem_bytes, uint64_t ddf_bytes) {
+	return !rows_on_lines(Nx, elem_bytes) || (k.stream_collide_linear_large && ddf_bytes > LARGE_DDF_BYTES);
 }
 
 // Kernel set of one (velocity set, storage, collision) variant; EXT = 0 instantiates the
@@ -193,7 +195,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static const LbmKernels* table() {
 		static const LbmKernels k = {initialize, stream_collide, update_fields, update_force_field, mass_partial, waves_cap,
 		                             stream_collide_record, adjoint_step, objective_partial, xcd_map, transfer_fi,
-		                             update_moving_boundaries, tune_waves_cap_large(Q, ST, EXT), linear_xcd};
+		                             update_moving_boundaries, tune_linear_large(Q, ST, EXT), linear_xcd};
 		return &k;
 	}
 };

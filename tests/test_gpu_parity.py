@@ -173,7 +173,8 @@ def test_full_size_256_two_steps(oracle):
 @pytest.mark.parametrize("storage,reps", [("FP32", (4, 8, 8)), ("FP16S", (8, 8, 8))])
 def test_lattice_beyond_4_gib_matches_its_tile(storage, reps):
     """64-bit addressing at full scale: a 512x512x256 FP32 / 512^3 FP16S D3Q19 SRT lattice
-    (5.1 GB of DDFs, slot offsets up to 4.8 GB; FP16S on the XCD-mapped grid) filled with a
+    (5.1 GB of DDFs, slot offsets up to 4.8 GB; FP32 above 4 GiB on the cell-linear grid,
+    FP16S on the XCD-mapped row grid, the 64^3 lattices on the row grid) filled with a
     Taylor-Green vortex of period 64 is the 64^3 vortex repeated, so after 3 steps every
     64^3 tile of slots 0, 9, 17 and 18 equals the 64^3 lattice's DDFs bit for bit (the 64^3
     lattice is oracle-checked above)."""

@@ -27,11 +27,13 @@ int main() {
   }
   printf("tune %u %u %u %u %u\n", tune_waves_cap(19, ST_FP32, 0), tune_waves_cap(27, ST_FP32, 0), tune_waves_cap(19, ST_FP16S, 0),
          tune_waves_cap(19, ST_FP32, EXT_EQUILIBRIUM_BOUNDARIES), tune_waves_cap(19, ST_FP32, EXT_ALL));
-  printf("large %u %u %u\n", tune_waves_cap_large(19, ST_FP32, 0), tune_waves_cap_large(27, ST_FP32, 0), tune_waves_cap_large(19, ST_FP16S, 0));
+  printf("large %d %d %d\n", (int)tune_linear_large(19, ST_FP32, 0), (int)tune_linear_large(27, ST_FP32, EXT_ALL), (int)tune_linear_large(19, ST_FP16S, 0));
   LbmKernels k{};
-  k.stream_collide_waves_cap = 3; k.stream_collide_waves_cap_large = 2;
-  printf("capsel %u %u %u %u\n", stream_waves_cap(k, LARGE_DDF_BYTES, false), stream_waves_cap(k, LARGE_DDF_BYTES + 1, false),
-         stream_waves_cap(k, LARGE_DDF_BYTES + 1, true), stream_waves_cap(k, 1, false));
+  k.stream_collide_linear_large = true;
+  printf("capsel %d %d %d %d\n", (int)use_linear_grid(k, 256, 4, LARGE_DDF_BYTES), (int)use_linear_grid(k, 256, 4, LARGE_DDF_BYTES + 1),
+         (int)use_linear_grid(k, 200, 4, 1), (int)use_linear_grid(k, 256, 4, 1));
+  k.stream_collide_linear_large = false;
+  printf("capsel16 %d %d\n", (int)use_linear_grid(k, 256, 2, LARGE_DDF_BYTES + 1), (int)use_linear_grid(k, 96, 2, 1));
   for (unsigned long long n : {1ull, 255ull, 256ull, 257ull, 8000000ull, 906000000ull}) for (int xcd = 0; xcd < 2; xcd++) {
     LaunchShape l = linear_shape(n, xcd != 0);
     printf("linear %llu %d %u %u %u %u %d\n", n, xcd, l.grid.x, l.grid.y, l.grid.z, l.block.x, (int)l.lin);
@@ -75,7 +77,7 @@ def launch_facts(tmp_path_factory):
             caps[(int(p[1]), int(p[2]))] = int(p[3])
         elif p[0] == "shape":
             shapes[int(p[1])] = tuple(int(v) for v in p[2:])
-        elif p[0] in ("tune", "large", "capsel"):
+        elif p[0] in ("tune", "large", "capsel", "capsel16"):
             tune = (tune or {}) | {p[0]: tuple(int(v) for v in p[1:])}
         elif p[0] == "lines":
             lines[int(p[1])] = (bool(int(p[2])), bool(int(p[3])))
@@ -120,10 +122,11 @@ def test_tuning_table(launch_facts):
     _, _, tune, _ = launch_facts
     # plain FP32 D3Q19/D3Q27 at 3 waves/SIMD, FP16 uncapped, EQUILIBRIUM_BOUNDARIES at 4, generic uncapped
     assert tune["tune"] == (3, 3, 0, 4, 0)
-    # above 4 GiB of DDFs plain D3Q19 FP32 drops to 2 waves/SIMD, the others keep their cap
-    assert tune["large"] == (2, 3, 0)
-    # the large-lattice cap applies strictly above the limit and only on the row grid
-    assert tune["capsel"] == (3, 2, 3, 3)
+    # above 4 GiB of DDFs the FP32 kernels take the cell-linear grid, the 16-bit ones keep rows
+    assert tune["large"] == (1, 1, 0)
+    # strictly above the limit, or wherever the rows miss the 128-byte lines
+    assert tune["capsel"] == (0, 1, 1, 0)
+    assert tune["capsel16"] == (0, 1)
 
 
 def test_grid_follows_row_alignment(launch_facts):
