@@ -94,6 +94,7 @@ fx::LbmArgs make_args(const fx_ctx* ctx, const void* fi, const float* rho, const
 	a.odd = (uint32_t)(t & 1ull);
 	a.xchunks = 0u; // set by the stream_collide launcher from its shape
 	a.w = ctx->cfg.w;
+	a.wm = fx::trt_minus_rate(a.w);
 	a.fx = fx_; a.fy = fy_; a.fz = fz_;
 	return a;
 }

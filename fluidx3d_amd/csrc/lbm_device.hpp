@@ -44,9 +44,15 @@ struct LbmArgs {
 	uint32_t ext;
 	uint32_t odd;   // parity of the time step t
 	float w;
+	float wm;       // TRT antisymmetric rate trt_minus_rate(w), formed once by the host
 	float fx, fy, fz;
 	uint32_t xchunks; // stream_collide on a 1-D XCD-mapped grid: x chunks per row (0: 3-D grid)
 };
+// TRT antisymmetric relaxation rate for magic parameter Lambda = 3/16 (src/kernel.cpp:
+// 1605-1633): the same IEEE float operations on the host and the device give the same
+// value, so the kernels take it from LbmArgs instead of three divisions per lane (SUBGRID,
+// whose rate varies per cell, forms it in the kernel).
+__host__ __device__ inline float trt_minus_rate(float w) { return 1.0f / (0.1875f / (1.0f / w - 0.5f) + 0.5f); }
 
 // ------------------------------------------------------------------------------------
 // DDF storage formats
@@ -497,7 +503,7 @@ template<bool TRT> constexpr bool uniform_rows(int Q, uint32_t ST, uint32_t EXT)
 	return ST == ST_FP32 ? TRT && !forced : forced;
 }
 template<int Q, bool TRT>
-__device__ __forceinline__ void collide_pairs(float w, bool vf, bool isE, const Equilibrium<Q>& eq, const Forcing<Q>& fo, float* fhn) {
+__device__ __forceinline__ void collide_pairs(float w, float wm, bool vf, bool isE, const Equilibrium<Q>& eq, const Forcing<Q>& fo, float* fhn) {
 	if constexpr(!TRT) { // src/kernel.cpp:1595-1604
 		const float c_tau = __builtin_fmaf(w, -0.5f, 1.0f);
 		const float omw = 1.0f - w;
@@ -516,7 +522,6 @@ __device__ __forceinline__ void collide_pairs(float w, bool vf, bool isE, const 
 		}
 	} else { // src/kernel.cpp:1605-1633, Lambda = 3/16
 		const float wp = w;
-		const float wm = 1.0f / (0.1875f / (1.0f / w - 0.5f) + 0.5f);
 		const float c_taup = __builtin_fmaf(wp, -0.25f, 0.5f), c_taum = __builtin_fmaf(wm, -0.25f, 0.5f);
 		const float hwp = 0.5f * wp, hwm = 0.5f * wm;
 		auto trt = [&](float fe, float feb, float fh, float fhb, float F) {
@@ -578,7 +583,7 @@ __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint3
 		a.rho[n] = rhon; a.u[n] = uxn; a.u[a.N + n] = uyn; a.u[2ull * a.N + n] = uzn;
 	}
 	if constexpr(collide_by_pairs(Q, TRT, EXT)) {
-		collide_pairs<Q, TRT>(a.w, vf, isE, Equilibrium<Q>(rhon, rhom1, uxn, uyn, uzn), Forcing<Q>(uxn, uyn, uzn, fxn, fyn, fzn), fhn);
+		collide_pairs<Q, TRT>(a.w, a.wm, vf, isE, Equilibrium<Q>(rhon, rhom1, uxn, uyn, uzn), Forcing<Q>(uxn, uyn, uzn, fxn, fyn, fzn), fhn);
 		return;
 	}
 	// whole arrays (SUBGRID needs every equilibrium population before the collision)
@@ -606,7 +611,7 @@ __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint3
 		}
 	} else { // src/kernel.cpp:1605-1633, Lambda = 3/16
 		const float wp = w;
-		const float wm = 1.0f / (0.1875f / (1.0f / w - 0.5f) + 0.5f);
+		const float wm = (a.ext & EXT_SUBGRID) ? trt_minus_rate(w) : a.wm;
 		if(vf) {
 			const float c_taup = __builtin_fmaf(wp, -0.25f, 0.5f), c_taum = __builtin_fmaf(wm, -0.25f, 0.5f);
 			float Fn[Q];
@@ -947,7 +952,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_step(AdjointArgs g) {
 	const float w = a.w;
 	float hp = 0.0f, hm = 0.0f, ctp = 0.0f, ctm = 0.0f;
 	if constexpr(TRT) {
-		const float wm = 1.0f / (0.1875f / (1.0f / w - 0.5f) + 0.5f);
+		const float wm = a.wm;
 		hp = 0.5f * w; hm = 0.5f * wm;
 		ctp = __builtin_fmaf(w, -0.25f, 0.5f); ctm = __builtin_fmaf(wm, -0.25f, 0.5f);
 	}
