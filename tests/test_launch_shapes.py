@@ -16,6 +16,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 PROGRAM = r'''
 #include "lbm_launch.hpp"
 #include <cstdio>
+#include <cstring>
 #include <vector>
 int main() {
   using namespace fx;
@@ -55,6 +56,13 @@ int main() {
   }
   for (unsigned nx : {20u, 32u, 64u, 150u, 200u, 256u, 300u, 320u, 384u, 520u})
     printf("lines %u %d %d\n", nx, (int)rows_on_lines(nx, 4), (int)rows_on_lines(nx, 2));
+  // TRT antisymmetric rate as the host forms it, for 2000 rates across (0, 2)
+  for (int k = 1; k <= 2000; k++) {
+    const float w = (float)k * 0.000999f;
+    const float wm = trt_minus_rate(w);
+    unsigned wb, mb; memcpy(&wb, &w, 4); memcpy(&mb, &wm, 4);
+    printf("trt %08x %08x\n", wb, mb);
+  }
   return 0;
 }
 '''
@@ -83,6 +91,8 @@ def launch_facts(tmp_path_factory):
             lines[int(p[1])] = (bool(int(p[2])), bool(int(p[3])))
         elif p[0] == "linear":
             lines[("linear", int(p[1]), int(p[2]))] = tuple(int(v) for v in p[3:])
+        elif p[0] == "trt":
+            lines.setdefault("trt", []).append((int(p[1], 16), int(p[2], 16)))
         elif p[0] == "segment":
             lines[("segment", int(p[1]))] = tuple(int(v) for v in p[2:])
     return caps, shapes, tune, lines
@@ -134,6 +144,8 @@ def test_grid_follows_row_alignment(launch_facts):
     blocks of 256 consecutive cells cover the lattice exactly once."""
     _, _, _, lines = launch_facts
     for key, v in lines.items():
+        if key == "trt":
+            continue
         if isinstance(key, tuple) and key[0] == "linear":
             n, xcd = key[1], key[2]
             gx, gy, gz, bx, lin = v
@@ -151,3 +163,17 @@ def test_grid_follows_row_alignment(launch_facts):
             fp32, fp16 = v
             assert fp32 == (key * 4 % 128 == 0) and fp16 == (key * 2 % 128 == 0), key
     assert lines[256] == (True, True) and lines[32] == (True, False) and lines[200] == (False, False)
+
+
+def test_trt_rate_formed_on_the_host(launch_facts):
+    """The kernels take the TRT antisymmetric rate 1/(0.1875/(1/w - 0.5) + 0.5) from the host
+    (src/kernel.cpp:1605-1633, Lambda = 3/16): the host's float operations must give the
+    bits that the same IEEE binary32 operations give, here evaluated with numpy float32."""
+    import numpy as np
+    pairs = launch_facts[3]["trt"]
+    assert len(pairs) == 2000
+    w = np.array([a for a, _ in pairs], np.uint32).view(np.float32)
+    f = np.float32
+    expect = (f(1.0) / (f(0.1875) / (f(1.0) / w - f(0.5)) + f(0.5))).astype(np.float32)
+    got = np.array([b for _, b in pairs], np.uint32).view(np.float32)
+    assert np.array_equal(got.view(np.uint32), expect.view(np.uint32))
