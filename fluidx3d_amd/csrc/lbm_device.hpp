@@ -510,6 +510,16 @@ template<bool TRT> constexpr bool uniform_rows(int Q, uint32_t ST, uint32_t EXT)
 	if(Q != 19) return false;
 	return ST == ST_FP32 ? TRT && !forced : forced;
 }
+// FP16S kernels that issue their DDF loads at raised wave priority (s_setprio 1, back to 0
+// once all are issued).  Measured at 256^3 against the same kernels without: plain D3Q19
+// +1.3 % (SRT) / +1.4 % (TRT), D3Q27 +1.3 % / +3.4 %; TRT VOLUME_FORCE +3.6 % (D3Q19), +8.5 %
+// (D3Q27); EQUILIBRIUM_BOUNDARIES (+FORCE_FIELD) -0.2 to +0.6 %; the porous and generic
+// kernels -1 to -5 % and SRT VOLUME_FORCE -0.8 % (D3Q19) go without; FP32 and FP16C lost
+// 3.5 % and 0.5 % with it.
+template<bool TRT> constexpr bool load_priority(uint32_t EXT) {
+	return EXT == 0u || EXT == EXT_EQUILIBRIUM_BOUNDARIES || EXT == (EXT_EQUILIBRIUM_BOUNDARIES | EXT_FORCE_FIELD) ||
+	       (TRT && EXT == EXT_VOLUME_FORCE);
+}
 template<int Q, bool TRT>
 __device__ __forceinline__ void collide_pairs(float w, float wm, bool vf, bool isE, const Equilibrium<Q>& eq, const Forcing<Q>& fo, float* fhn) {
 	if constexpr(!TRT) { // src/kernel.cpp:1595-1604
@@ -726,8 +736,10 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 		// every load is issued before the first decode: left alone, the scheduler may
 		// convert each 16-bit value right after its load and wait for it, one at a time
 		E r[Q];
+		if constexpr(load_priority<TRT>(EXT)) __builtin_amdgcn_s_setprio(1);
 		load_raw<Q, ST, U, LIN>(fi, a.N, s, a.odd, r);
 		__builtin_amdgcn_sched_barrier(0);
+		if constexpr(load_priority<TRT>(EXT)) __builtin_amdgcn_s_setprio(0);
 #pragma unroll
 		for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
 	} else {
