@@ -501,14 +501,26 @@ template<int Q> __device__ __forceinline__ float subgrid_rate(float w, const flo
 constexpr bool collide_by_pairs(int Q, bool TRT, uint32_t EXT) {
 	return (EXT & EXT_SUBGRID) == 0u && (EXT & (EXT_VOLUME_FORCE | EXT_POROSITY)) != 0u && (TRT || Q == 19);
 }
+// kernels_for selects the VOLUME_FORCE subset only when exactly that extension is on, so
+// the mask can be a compile-time constant there (the unforced paths compile out).  Measured
+// at 256^3: D3Q27 FP16S/FP16C +9 to +18 %; on the D3Q19 row grid, with plain row bases
+// (uniform_rows), FP16S TRT +5.7 %, FP16C SRT +3.1 %, FP16C TRT +8.4 %, FP16S SRT -2 % (it
+// keeps the runtime mask there); on the D3Q19 cell-linear grid (200^3) SRT +26 % (FP16S)
+// and +21.5 % (FP16C, 98 -> 75-79 VGPRs), TRT -1.3 to +0.6 %; FP32 -5 to +1.5 %.
+template<bool TRT> constexpr bool constant_vf_mask(int Q, uint32_t ST, uint32_t EXT, bool LIN) {
+	if(ST == ST_FP32 || EXT != EXT_VOLUME_FORCE) return false;
+	return Q == 27 || LIN || TRT || ST == ST_FP16C;
+}
 // Uniform row bases (readfirstlane) for stream_collide: the forced D3Q19 16-bit kernels
 // drop from 96-99 to 58-66 VGPRs and gain 2-20 %, the unforced D3Q19 FP32 TRT kernels
 // 2-4 %; D3Q27 (27 slot bases in scalar registers) and the unforced 16-bit kernels lost up
-// to 30 %, the forced FP32 ones up to 1 %, so they keep the plain bases.
+// to 30 %, the forced FP32 ones up to 1 %, so they keep the plain bases.  The 16-bit
+// VOLUME_FORCE kernels with a constant extension mask (constant_vf_mask) hold 59-63 VGPRs
+// with plain bases as well and need fewer instructions that way.
 template<bool TRT> constexpr bool uniform_rows(int Q, uint32_t ST, uint32_t EXT) {
 	const bool forced = (EXT & (EXT_VOLUME_FORCE | EXT_POROSITY)) != 0u;
 	if(Q != 19) return false;
-	return ST == ST_FP32 ? TRT && !forced : forced;
+	return ST == ST_FP32 ? TRT && !forced : forced && !constant_vf_mask<TRT>(Q, ST, EXT, false);
 }
 // FP16S kernels that issue their DDF loads at raised wave priority (s_setprio 1, back to 0
 // once all are issued).  Measured at 256^3 against the same kernels without: plain D3Q19
@@ -716,11 +728,7 @@ template<bool XCD, bool LIN> __device__ __forceinline__ bool cell_coords(const L
 template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD, bool XCD, bool LIN>
 __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 	a.ext &= EXT;
-	// kernels_for selects the VOLUME_FORCE subset only when exactly that extension is on, so
-	// the mask can be a compile-time constant there (the unforced paths compile out).
-	// Measured at 256^3: D3Q27 FP16S/FP16C +9 to +18 %; D3Q19 16-bit -7 to -14 % (58-64 ->
-	// 76-79 VGPRs, 8 -> 6 waves/SIMD), FP32 -5 to +1.5 %, so only D3Q27 16-bit takes it.
-	if constexpr(Q == 27 && ST != ST_FP32 && EXT == EXT_VOLUME_FORCE) a.ext = EXT_VOLUME_FORCE;
+	if constexpr(constant_vf_mask<TRT>(Q, ST, EXT, LIN)) a.ext = EXT_VOLUME_FORCE;
 	uint32_t x, y, z;
 	if(!cell_coords<XCD, LIN>(a, x, y, z)) return;
 	constexpr bool U = !LIN && uniform_rows<TRT>(Q, ST, EXT); // rows are per lane on the cell-linear grid
