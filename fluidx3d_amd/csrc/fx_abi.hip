@@ -415,7 +415,7 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 	// cell-linear grid for rows that miss the 128-byte lines and for large FP32 lattices
 	// (the adjoint's FP32 lambda rows are aligned whenever the DDF rows are)
 	const uint32_t eb = cfg->storage == FX_FP32 ? 4u : 2u;
-	if(fx::use_linear_grid(*ctx->k, cfg->Nx, eb, N * cfg->velocity_set * eb)) {
+	if(fx::use_linear_grid(*ctx->k, cfg->Nx, eb, N, cfg->velocity_set)) {
 		ctx->shape_sc = fx::linear_shape(N, ctx->k->stream_collide_linear_xcd);
 		ctx->shape_adj = fx::linear_shape(N, false);
 	}

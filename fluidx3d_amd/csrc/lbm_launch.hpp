@@ -139,9 +139,12 @@ constexpr bool tune_linear_large(int Q, uint32_t ST, uint32_t EXT) { return ST =
 // 1000x250x250, full occupancy) and -2 to +1 % for FP32 at 3 waves/SIMD.
 constexpr bool tune_linear_xcd(int Q, uint32_t ST, uint32_t EXT) { return ST != ST_FP32; }
 // stream_collide (and the adjoint kernels) on the cell-linear grid: rows that miss the
-// 128-byte lines, or a large FP32 lattice.
-inline bool use_linear_grid(const LbmKernels& k, uint32_t Nx, uint32_t elem_bytes, uint64_t ddf_bytes) {
-	return !rows_on_lines(Nx, elem_bytes) || (k.stream_collide_linear_large && ddf_bytes > LARGE_DDF_BYTES);
+// 128-byte lines, or a large FP32 lattice.  The 16-bit kernels address a slot there with
+// 32-bit offsets (slot_offsets32), so a 16-bit slot of 4 GiB or more (2^31 cells) keeps
+// the row grid.
+inline bool use_linear_grid(const LbmKernels& k, uint32_t Nx, uint32_t elem_bytes, uint64_t N, uint32_t q) {
+	if(elem_bytes != 4u && N * elem_bytes > 0xFFFFFFFFull) return false;
+	return !rows_on_lines(Nx, elem_bytes) || (k.stream_collide_linear_large && N * q * elem_bytes > LARGE_DDF_BYTES);
 }
 
 // Kernel set of one (velocity set, storage, collision) variant; EXT = 0 instantiates the

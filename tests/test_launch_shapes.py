@@ -31,10 +31,14 @@ int main() {
   printf("large %d %d %d\n", (int)tune_linear_large(19, ST_FP32, 0), (int)tune_linear_large(27, ST_FP32, EXT_ALL), (int)tune_linear_large(19, ST_FP16S, 0));
   LbmKernels k{};
   k.stream_collide_linear_large = true;
-  printf("capsel %d %d %d %d\n", (int)use_linear_grid(k, 256, 4, LARGE_DDF_BYTES), (int)use_linear_grid(k, 256, 4, LARGE_DDF_BYTES + 1),
-         (int)use_linear_grid(k, 200, 4, 1), (int)use_linear_grid(k, 256, 4, 1));
+  // N*q*4 bytes at and just above the 4 GiB limit (q = 16 keeps the arithmetic exact)
+  const uint64_t n_lim = LARGE_DDF_BYTES / 64;
+  printf("capsel %d %d %d %d\n", (int)use_linear_grid(k, 256, 4, n_lim, 16), (int)use_linear_grid(k, 256, 4, n_lim + 1, 16),
+         (int)use_linear_grid(k, 200, 4, 1000, 19), (int)use_linear_grid(k, 256, 4, 1000, 19));
   k.stream_collide_linear_large = false;
-  printf("capsel16 %d %d\n", (int)use_linear_grid(k, 256, 2, LARGE_DDF_BYTES + 1), (int)use_linear_grid(k, 96, 2, 1));
+  // 16-bit: rows off the lines take the cell-linear grid unless a slot reaches 4 GiB
+  printf("capsel16 %d %d %d %d\n", (int)use_linear_grid(k, 256, 2, 1ull << 30, 19), (int)use_linear_grid(k, 96, 2, 1000, 19),
+         (int)use_linear_grid(k, 96, 2, (1ull << 31) - 1, 19), (int)use_linear_grid(k, 96, 2, 1ull << 31, 19));
   for (unsigned long long n : {1ull, 255ull, 256ull, 257ull, 8000000ull, 906000000ull}) for (int xcd = 0; xcd < 2; xcd++) {
     LaunchShape l = linear_shape(n, xcd != 0);
     printf("linear %llu %d %u %u %u %u %d\n", n, xcd, l.grid.x, l.grid.y, l.grid.z, l.block.x, (int)l.lin);
@@ -136,7 +140,7 @@ def test_tuning_table(launch_facts):
     assert tune["large"] == (1, 1, 0)
     # strictly above the limit, or wherever the rows miss the 128-byte lines
     assert tune["capsel"] == (0, 1, 1, 0)
-    assert tune["capsel16"] == (0, 1)
+    assert tune["capsel16"] == (0, 1, 1, 0)
 
 
 def test_grid_follows_row_alignment(launch_facts):
