@@ -351,7 +351,8 @@ __device__ __forceinline__ Site make_site(uint32_t x, uint32_t y, uint32_t z, ui
 // element offset (a slot is below 4 GiB wherever the cell-linear grid runs them,
 // use_linear_grid): one register per address instead of a 64-bit pair, one for all
 // unshifted slots; 69-71 instead of 79 VGPRs for the plain kernels, measured +0.4 to
-// +7.6 % on 14 of the 16 D3Q19 16-bit variants at 200^3 (FP32: -2.3 to +1 %, not taken).
+// +7.6 % on 14 of the 16 D3Q19 16-bit variants at 200^3, D3Q27 -4.8 to +18 % (11 of 16
+// up); FP32 -2.3 to +1 %, not taken.
 template<int Q, uint32_t ST> constexpr bool slot_offsets32(bool SC) { return SC && ST != ST_FP32; }
 template<int Q, uint32_t ST, bool U = false, bool SC = false>
 __device__ __forceinline__ void load_raw(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, typename Storage<ST>::E* r) {
