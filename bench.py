@@ -5,11 +5,18 @@ Prints ONE JSON line.  `value` is MLUPs/s of the FP32-storage run (the reference
 headline configuration, src/setup.cpp:5-36 with README.md:1137); FP16S and FP16C are
 measured the same way and reported under "modes".
 
-Timing: W untimed warm-up steps, then exactly K steps launched back to back on the
-context stream and bracketed by HIP events on that stream (fx_lbm_run), with a device
-synchronisation before and after.  MLUPs/s = N*K/t.  roofline.achieved = algorithmic
-bytes per launch (bandwidth_bytes_per_cell_device, src/lbm.cpp:51-72: 153 B FP32, 77 B
-FP16 per cell) / average launch duration; peak = 8000 GB/s (MI355X).
+Timing, per storage mode: W untimed warm-up steps, then the reference-style figure (one
+launch and one blocking synchronisation per step, src/lbm.cpp:951, 964-973), which also
+brings the clocks to steady state, then `--repeats` batches of exactly K steps launched
+back to back on the context stream and bracketed by HIP events on that stream
+(fx_lbm_run), with a device synchronisation before and after each.  MLUPs/s = N*K/t of
+the median batch (min and max reported beside it).  roofline.achieved = algorithmic bytes
+per launch (bandwidth_bytes_per_cell_device, src/lbm.cpp:51-72: 153 B FP32, 77 B FP16
+per cell) / the median batch's average launch duration; peak = 8000 GB/s (MI355X).
+The device copy peak is measured first.  Beside the headline: the two sphere-flow
+lattices (512x256x256 D3Q27 TRT FP16C with equilibrium boundaries and the force field,
+1536x768x768 D3Q19 SRT FP16S), the checkpointed adjoint against the plain forward step,
+and the CPU baseline.
 """
 import argparse
 import json
@@ -31,76 +38,166 @@ REFERENCE_MLUPS = {"FP32": 30380.0, "FP16S": 54494.0, "FP16C": 40157.0}  # READM
 def parse_args():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10000)
-    p.add_argument("--warmup", type=int, default=200)
+    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--warmup", type=int, default=100)
+    p.add_argument("--repeats", type=int, default=3, help="event-timed batches of K steps per mode (median reported)")
     p.add_argument("--size", type=int, default=256)
     p.add_argument("--modes", default="FP32,FP16S,FP16C")
     p.add_argument("--sync-steps", type=int, default=200, help="steps of the per-step-synchronised figure")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-adjoint", action="store_true")
+    p.add_argument("--no-spheres", action="store_true", help="skip the sphere-flow legs (512x256x256 D3Q27, 1536x768x768)")
     return p.parse_args()
 
 
-def read_traffic(mode):
-    """HBM bytes per stream_collide launch from the committed rocprofv3 --pmc summary."""
+def read_traffic(mode, size):
+    """HBM bytes per stream_collide launch from the committed rocprofv3 --pmc summary, which
+    holds them for one lattice size (lines `traffic_bytes_per_launch[MODE@S^3] = bytes`);
+    None for a size it does not cover."""
     path = os.path.join(ROOT, "profiles", "stream_collide_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        m = re.match(r"\s*traffic_bytes_per_launch\[(\w+)\]\s*=\s*([0-9.eE+]+)", line)
-        if m and m.group(1) == mode:
-            return float(m.group(2))
+        m = re.match(r"\s*traffic_bytes_per_launch\[(\w+)@(\d+)\^3\]\s*=\s*([0-9.eE+]+)", line)
+        if m and m.group(1) == mode and int(m.group(2)) == size:
+            return float(m.group(3))
     return None
 
 
-def gpu_mode(fx, size, storage, steps, warmup, sync_steps):
-    from tests import scenes
+def timed_batches(L, args, t, steps, repeats, fx_=0.0, F=None, alpha=None):
+    """`repeats` batches of exactly `steps` stream_collide launches back to back on the
+    context stream, each bracketed by HIP events on that stream (fx_lbm_run); returns the
+    per-batch milliseconds and the next t."""
+    import ctypes
     from fluidx3d_amd import _abi
+    ms = ctypes.c_float(0.0)
+    out = []
+    for _ in range(repeats):
+        _abi.check(L.fx_lbm_run(*args, t, steps, fx_, 0.0, 0.0, F, alpha, ctypes.byref(ms)))
+        out.append(ms.value)
+        t += steps
+    return out, t
+
+
+def sync_leg(L, ctx, args, t, steps, F=None):
+    """Reference-style figure: one launch + one blocking synchronisation per step
+    (src/lbm.cpp:951, 964-973); seconds per step and the next t."""
+    from fluidx3d_amd import _abi
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _abi.check(L.fx_lbm_stream_collide(*args, t, 0.0, 0.0, 0.0, F, None))
+        ctx.sync()
+        t += 1
+    return (time.perf_counter() - t0) / max(steps, 1), t
+
+
+def rate_summary(N, bpc, batch_ms, steps):
+    """MLUPs/s of the median batch (min and max beside it) and its roofline figures."""
+    per_launch = sorted(b / steps for b in batch_ms)
+    med = per_launch[len(per_launch) // 2]
+    gbps = bpc * N / (med * 1e-3) * 1e-9
+    return {
+        "mlups": N / (med * 1e-3) * 1e-6,
+        "mlups_min": N / (per_launch[-1] * 1e-3) * 1e-6,
+        "mlups_max": N / (per_launch[0] * 1e-3) * 1e-6,
+        "ms_per_step": med,
+        "batches": len(per_launch),
+        "bytes_per_cell": bpc,
+        "achieved_GBps": gbps,
+        "frac": gbps / PEAK_HBM_GBPS,
+    }
+
+
+def gpu_mode(fx, size, storage, steps, warmup, sync_steps, repeats):
+    """One storage mode of the 256^3 Taylor-Green benchmark.  Order: initialize, W warm-up
+    steps, the per-step-synchronised leg (which also brings the clocks to steady state),
+    then `repeats` event-timed batches of exactly K steps; the median batch is the rate."""
+    from tests import scenes
     lbm = fx.LBM(size, size, size, 1.0 / 6.0, storage=storage)
     rho, u, flags = scenes.taylor_green(size, size, size)
     lbm.rho.host[:] = rho
     lbm.u.host[:] = u
     lbm.flags.host[:] = flags
     lbm.initialize()
+    m0 = lbm.mass()
     L = lbm.ctx.lib
-    import ctypes
-    ms = ctypes.c_float(0.0)
     args = (lbm.ctx.h, lbm.fi, lbm.rho.device, lbm.u.device, lbm.flags.device)
-    _abi.check(L.fx_lbm_run(*args, 0, warmup, 0.0, 0.0, 0.0, None, None, ctypes.byref(ms)))
-    t = warmup
+    _, t = timed_batches(L, args, 0, warmup, 1)
+    sync_s, t = sync_leg(L, lbm.ctx, args, t, sync_steps)
     lbm.ctx.sync()
     wall0 = time.perf_counter()
-    _abi.check(L.fx_lbm_run(*args, t, steps, 0.0, 0.0, 0.0, None, None, ctypes.byref(ms)))
+    batch_ms, t = timed_batches(L, args, t, steps, repeats)
     lbm.ctx.sync()
     wall = time.perf_counter() - wall0
-    t += steps
-    event_s = ms.value * 1e-3
+    lbm.t = t
     N = lbm.N
-    bpc = fx.bandwidth_bytes_per_cell(lbm.cfg)
-    # reference-style figure: one launch + one blocking synchronisation per step (src/lbm.cpp:951, 964-973)
-    t0 = time.perf_counter()
-    for _ in range(sync_steps):
-        _abi.check(L.fx_lbm_stream_collide(*args, t, 0.0, 0.0, 0.0, None, None))
-        lbm.ctx.sync()
-        t += 1
-    sync_s = (time.perf_counter() - t0) / sync_steps
-    mass = lbm.mass()
+    out = rate_summary(N, fx.bandwidth_bytes_per_cell(lbm.cfg), batch_ms, steps)
+    out["wall_mlups"] = N * steps * repeats / wall * 1e-6
+    out["per_step_sync_mlups"] = N / sync_s * 1e-6
+    out["mass_rel_drift"] = f"{abs(lbm.mass() - m0) / m0:.3e}"
     lbm.u.read_from_device()
-    finite = bool(np.isfinite(lbm.u.host).all())
+    out["finite"] = bool(np.isfinite(lbm.u.host).all())
     lbm.close()
-    per_launch = event_s / steps
-    return {
-        "mlups": N * steps / event_s * 1e-6,
-        "ms_per_step": per_launch * 1e3,
-        "wall_mlups": N * steps / wall * 1e-6,
-        "bytes_per_cell": bpc,
-        "achieved_GBps": bpc * N / per_launch * 1e-9,
-        "frac": bpc * N / per_launch * 1e-9 / PEAK_HBM_GBPS,
-        "per_step_sync_mlups": N / sync_s * 1e-6,
-        "mass_rel_drift": abs(mass - N) / N,
-        "finite": finite,
-    }
+    return out
+
+
+def sphere_leg(fx, name, dims, radius, velocity_set, collision, storage, extensions, steps, warmup, sync_steps, repeats,
+               perturb):
+    """Flow past a sphere (the C++ scene of fluidx3d_amd/host/setup.cpp): TYPE_S|TYPE_X
+    sphere() of radius R at (R*4, R*4, R*4), TYPE_E inflow/outflow u = (0.075, 0, 0) at
+    x = 0 and x = Nx-1, periodic in y and z, nu for Re = 100 on the diameter; `perturb`
+    adds a seeded uniform perturbation of amplitude 1e-4 (numpy PCG64, seed 42) to u of the
+    other cells.  Timed like the benchmark (the drag after 20000 steps is a test, not a
+    bench figure)."""
+    from tests import scenes
+    Nx, Ny, Nz = dims
+    u_in = 0.075
+    nu = u_in * 2.0 * radius / 100.0
+    lbm = fx.LBM(Nx, Ny, Nz, nu, velocity_set=velocity_set, collision=collision, storage=storage,
+                 extensions=extensions)
+    N = lbm.N
+    c = 4.0 * radius
+    plane = Nx * Ny
+    rng = np.random.default_rng(42) if perturb else None
+    x = np.arange(Nx, dtype=np.float32) - np.float32(c)
+    y = np.arange(Ny, dtype=np.float32) - np.float32(c)
+    dxy = (x * x)[None, :] + (y * y)[:, None]
+    r2 = np.float32(radius) * np.float32(radius)
+    lbm.u.x[:] = u_in
+    for z in range(Nz):  # plane by plane: 906M-cell lattices stay within a few GB of temporaries
+        sl = slice(z * plane, (z + 1) * plane)
+        dz = np.float32(z) - np.float32(c)
+        solid = (dxy + dz * dz <= r2).ravel()
+        f = np.zeros(plane, dtype=np.uint8)
+        f[solid] = scenes.TYPE_S | scenes.TYPE_X
+        edge = np.zeros((Ny, Nx), dtype=bool)
+        edge[:, 0] = edge[:, -1] = True
+        edge = edge.ravel() & ~solid
+        f[edge] = scenes.TYPE_E
+        lbm.flags.host[sl] = f
+        if perturb:
+            p = rng.uniform(-1e-4, 1e-4, (3, plane)).astype(np.float32)
+            keep = ~(solid | edge)
+            lbm.u.x[sl][keep] += p[0][keep]
+            lbm.u.y[sl][keep] = p[1][keep]
+            lbm.u.z[sl][keep] = p[2][keep]
+        lbm.u.x[sl][solid] = 0.0
+    lbm.initialize()
+    L = lbm.ctx.lib
+    args = (lbm.ctx.h, lbm.fi, lbm.rho.device, lbm.u.device, lbm.flags.device)
+    F = lbm.F.device if lbm.F is not None else None
+    _, t = timed_batches(L, args, 0, warmup, 1, F=F)
+    sync_s, t = sync_leg(L, lbm.ctx, args, t, sync_steps, F=F)
+    batch_ms, t = timed_batches(L, args, t, steps, repeats, F=F)
+    lbm.t = t
+    out = {"workload": name}
+    out.update(rate_summary(N, fx.bandwidth_bytes_per_cell(lbm.cfg), batch_ms, steps))
+    out["per_step_sync_mlups"] = N / sync_s * 1e-6
+    out["steps_run"] = t
+    lbm.close()
+    return out
 
 
 def adjoint_timing(fx, dims, steps, window, interval):
@@ -199,12 +296,30 @@ def main():
         sys.exit(2)
     import fluidx3d_amd as fx
     modes = [m.strip() for m in a.modes.split(",") if m.strip()]
+    copy_gbps = measured_copy_peak(fx)
+    print(f"# device copy: {copy_gbps:.0f} GB/s", file=sys.stderr, flush=True)
     results = {}
     for m in modes:
-        results[m] = gpu_mode(fx, a.size, m, a.steps, a.warmup, a.sync_steps)
-        print(f"# {m}: {results[m]['mlups']:.0f} MLUPs/s, {results[m]['achieved_GBps']:.0f} GB/s "
-              f"({100 * results[m]['frac']:.1f}% of 8 TB/s)", file=sys.stderr, flush=True)
-    copy_gbps = measured_copy_peak(fx)
+        results[m] = gpu_mode(fx, a.size, m, a.steps, a.warmup, a.sync_steps, a.repeats)
+        r = results[m]
+        print(f"# {m}: {r['mlups']:.0f} MLUPs/s (batches {r['mlups_min']:.0f}-{r['mlups_max']:.0f}), "
+              f"{r['achieved_GBps']:.0f} GB/s ({100 * r['frac']:.1f}% of 8 TB/s); per-step-synchronised "
+              f"{r['per_step_sync_mlups']:.0f}", file=sys.stderr, flush=True)
+    spheres = None
+    if not a.no_spheres:
+        spheres = {}
+        legs = [("sphere_512x256x256", "D3Q27 TRT FP16C flow past a sphere, 512x256x256, R=32, TYPE_E inflow/outflow "
+                 "u=0.075, Re=100, EQUILIBRIUM_BOUNDARIES+FORCE_FIELD", (512, 256, 256), 32.0, 27, "TRT", "FP16C",
+                 ("EQUILIBRIUM_BOUNDARIES", "FORCE_FIELD"), min(a.steps, 1000), True),
+                ("sphere_1536x768x768", "D3Q19 SRT FP16S flow past a sphere, 1536x768x768 (906M cells), R=96, TYPE_E "
+                 "inflow/outflow u=0.075, Re=100, EQUILIBRIUM_BOUNDARIES", (1536, 768, 768), 96.0, 19, "SRT", "FP16S",
+                 ("EQUILIBRIUM_BOUNDARIES",), min(a.steps, 200), False)]
+        for key, name, dims, radius, q, coll, st, ext, steps, perturb in legs:
+            r = sphere_leg(fx, name, dims, radius, q, coll, st, ext, steps, min(a.warmup, 20), min(a.sync_steps, 50),
+                           a.repeats, perturb)
+            spheres[key] = {k: (float(f"{v:.6g}") if isinstance(v, float) else v) for k, v in r.items()}
+            print(f"# {key}: {r['mlups']:.0f} MLUPs/s, {r['achieved_GBps']:.0f} GB/s ({100 * r['frac']:.1f}% of 8 TB/s)",
+                  file=sys.stderr, flush=True)
     adjoint = None
     if not a.no_adjoint:
         adjoint = {"scaled": adjoint_timing(fx, (384, 192, 192), 5000, 3750, 0),
@@ -213,7 +328,7 @@ def main():
               file=sys.stderr, flush=True)
     primary = "FP32" if "FP32" in results else modes[0]
     r = results[primary]
-    traffic = read_traffic(primary)
+    traffic = read_traffic(primary, a.size)
     out = {
         "metric": "MLUPs/s + %HBM roofline, D3Q19 FP32/FP16S 256^3; adjoint/forward time",
         "value": round(r["mlups"], 1),
@@ -234,8 +349,9 @@ def main():
                      "frac": round(r["frac"], 4), "traffic": traffic},
         "measured_copy_GBps": round(copy_gbps, 1),
         "frac_of_measured_copy": round(r["achieved_GBps"] / copy_gbps, 4),
-        "modes": {m: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in results[m].items()} for m in results},
+        "modes": {m: {k: (float(f"{v:.6g}") if isinstance(v, float) else v) for k, v in results[m].items()} for m in results},
         "reference_mlups_mi355x": REFERENCE_MLUPS,
+        "spheres": spheres,
         "adjoint": adjoint,
     }
     if not a.no_cpu:

@@ -10,6 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libfx_lbm.so")
 
 FX_OK = 0
+FX_ERR_ARG, FX_ERR_HIP, FX_ERR_CONFIG, FX_ERR_MEMORY, FX_ERR_STATE = -1, -2, -3, -4, -5
 FX_D3Q19, FX_D3Q27 = 19, 27
 FX_SRT, FX_TRT = 0, 1
 FX_FP32, FX_FP16S, FX_FP16C = 0, 1, 2

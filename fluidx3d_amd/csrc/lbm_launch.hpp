@@ -11,6 +11,11 @@ struct LaunchShape {
 	uint32_t xchunks = 0; // 1-D XCD-mapped grid with this many x chunks per row (see block_coords)
 	bool lin = false;     // cell-linear grid (cell_coords in lbm_device.hpp), see linear_shape
 };
+// A launch holds at most 2^32-1 work-items along each grid dimension (grid size times
+// block size); lattices of more than ~4.29 G cells (a D3Q19 FP16 lattice of up to ~5.2 G
+// cells fits in 288 GB) must not take a 1-D grid over all cells.
+constexpr uint64_t MAX_ITEMS_PER_DIMENSION = 0xFFFFFFFFull;
+inline bool grid_fits(uint64_t blocks, uint32_t block_threads) { return blocks * block_threads <= MAX_ITEMS_PER_DIMENSION; }
 // Every row of every DDF slot starts on a 128-byte line when Nx times the element size is
 // a multiple of 128 bytes (the slot stride N times the element size then is one too, and
 // device buffers are allocated 256-byte aligned).
@@ -20,10 +25,13 @@ inline bool rows_on_lines(uint32_t Nx, uint32_t elem_bytes) { return (uint64_t)N
 // rows with cached accesses): 200^3 FP32 +21 %, FP16S +24 %; 300^3 +16 %, +23 %;
 // 1000x250x250 +12 %, +3 %; 352x352x256 FP16S +19 %.  xcd pads the grid to whole groups
 // of 64 blocks for the XCD-aware segment order (linear_segment).
+inline uint64_t linear_blocks(uint64_t N, bool xcd) {
+	const uint64_t blocks = (N + 255u) / 256u;
+	return xcd ? (blocks + 63u) / 64u * 64u : blocks;
+}
 inline LaunchShape linear_shape(uint64_t N, bool xcd) {
 	LaunchShape s;
-	uint64_t blocks = (N + 255u) / 256u;
-	if(xcd) blocks = (blocks + 63u) / 64u * 64u;
+	const uint64_t blocks = linear_blocks(N, xcd);
 	s.block = dim3(256, 1, 1);
 	s.grid = dim3((unsigned)blocks, 1, 1);
 	s.lin = true;
@@ -48,7 +56,8 @@ inline LaunchShape stream_shape(uint32_t Nx, uint32_t Ny, uint32_t Nz, bool xcd_
 	LaunchShape s = lattice_shape(Nx, Ny, Nz);
 	const uint32_t chunks = s.grid.x;
 	const uint64_t groups = ((uint64_t)Ny * Nz + 7u) / 8u;
-	if(xcd_map && chunks > 1u && groups * 8u * chunks < 0x7FFFFFFFull) {
+	// otherwise the 3-D row grid: (chunks, Ny, Nz) stays within every dimension's limit
+	if(xcd_map && chunks > 1u && grid_fits(groups * 8u * chunks, s.block.x)) {
 		s.grid = dim3((unsigned)(groups * 8u * chunks), 1, 1);
 		s.xchunks = chunks;
 	}
@@ -144,6 +153,7 @@ constexpr bool tune_linear_xcd(int Q, uint32_t ST, uint32_t EXT) { return ST != 
 // the row grid.
 inline bool use_linear_grid(const LbmKernels& k, uint32_t Nx, uint32_t elem_bytes, uint64_t N, uint32_t q) {
 	if(elem_bytes != 4u && N * elem_bytes > 0xFFFFFFFFull) return false;
+	if(!grid_fits(linear_blocks(N, k.stream_collide_linear_xcd), 256u)) return false;
 	return !rows_on_lines(Nx, elem_bytes) || (k.stream_collide_linear_large && N * q * elem_bytes > LARGE_DDF_BYTES);
 }
 

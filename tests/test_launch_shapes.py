@@ -60,6 +60,22 @@ int main() {
   }
   for (unsigned nx : {20u, 32u, 64u, 150u, 200u, 256u, 300u, 320u, 384u, 520u})
     printf("lines %u %d %d\n", nx, (int)rows_on_lines(nx, 4), (int)rows_on_lines(nx, 2));
+  // lattices of up to ~5.2 G cells (D3Q19 FP16 in 288 GB): every 1-D grid within 2^32-1
+  // work-items, else the 3-D row grid
+  const unsigned big[][3] = {{1536u, 768u, 768u}, {2048u, 1536u, 1536u}, {4096u, 1024u, 1024u}, {256u, 4096u, 4096u},
+                             {1024u, 2048u, 2048u}, {512u, 65536u, 128u}};
+  for (const auto& d : big) {
+    LaunchShape m = stream_shape(d[0], d[1], d[2], true);
+    printf("big %u %u %u %u %u %u %u %u\n", d[0], d[1], d[2], m.grid.x, m.grid.y, m.grid.z, m.block.x, m.xchunks);
+    const unsigned long long n = (unsigned long long)d[0] * d[1] * d[2];
+    LbmKernels kl{};
+    kl.stream_collide_linear_large = true;
+    kl.stream_collide_linear_xcd = true;
+    // rows off the lines (Nx+1) and the large-FP32 rule, with the 1-D linear grid's size
+    printf("biglin %u %u %u %d %d %llu\n", d[0], d[1], d[2], (int)use_linear_grid(kl, d[0] + 1u, 4, n, 19),
+           (int)use_linear_grid(kl, d[0], 4, n, 19), (unsigned long long)linear_blocks(n, true));
+  }
+  printf("fits %d %d %d\n", (int)grid_fits(16777215ull, 256u), (int)grid_fits(16777216ull, 256u), (int)grid_fits(4294967295ull, 1u));
   // TRT antisymmetric rate as the host forms it, for 2000 rates across (0, 2)
   for (int k = 1; k <= 2000; k++) {
     const float w = (float)k * 0.000999f;
@@ -99,6 +115,10 @@ def launch_facts(tmp_path_factory):
             lines.setdefault("trt", []).append((int(p[1], 16), int(p[2], 16)))
         elif p[0] == "segment":
             lines[("segment", int(p[1]))] = tuple(int(v) for v in p[2:])
+        elif p[0] in ("big", "biglin"):
+            lines.setdefault(p[0], []).append(tuple(int(v) for v in p[1:]))
+        elif p[0] == "fits":
+            lines["fits"] = tuple(int(v) for v in p[1:])
     return caps, shapes, tune, lines
 
 
@@ -148,7 +168,7 @@ def test_grid_follows_row_alignment(launch_facts):
     blocks of 256 consecutive cells cover the lattice exactly once."""
     _, _, _, lines = launch_facts
     for key, v in lines.items():
-        if key == "trt":
+        if key in ("trt", "big", "biglin", "fits"):
             continue
         if isinstance(key, tuple) and key[0] == "linear":
             n, xcd = key[1], key[2]
@@ -181,3 +201,25 @@ def test_trt_rate_formed_on_the_host(launch_facts):
     expect = (f(1.0) / (f(0.1875) / (f(1.0) / w - f(0.5)) + f(0.5))).astype(np.float32)
     got = np.array([b for _, b in pairs], np.uint32).view(np.float32)
     assert np.array_equal(got.view(np.uint32), expect.view(np.uint32))
+
+
+def test_large_lattices_stay_within_the_launch_limit(launch_facts):
+    """A launch holds at most 2^32-1 work-items per grid dimension.  Above that the XCD-mapped
+    1-D grid falls back to the 3-D row grid (chunks, Ny, Nz), which covers every cell, and the
+    cell-linear grid is not taken; below it both are kept (906M cells: XCD-mapped)."""
+    lines = launch_facts[3]
+    assert lines["fits"] == (1, 0, 1)
+    limit = 2 ** 32 - 1
+    for Nx, Ny, Nz, gx, gy, gz, bx, chunks in lines["big"]:
+        row_chunks = -(-Nx // bx)
+        assert gx * bx <= limit and gy <= 65536 and gz <= 65536
+        if chunks:  # XCD-mapped 1-D grid over every row
+            assert (gy, gz) == (1, 1) and chunks == row_chunks and gx == -(-(Ny * Nz) // 8) * 8 * chunks
+        else:
+            assert (gx, gy, gz) == (row_chunks, Ny, Nz)
+        if Nx * Ny * Nz <= 906_000_000:
+            assert chunks, "lattices below the limit keep the XCD-mapped grid"
+    assert sum(1 for b in lines["big"] if b[7] == 0) >= 3  # the ones above 4.29 G cells fall back
+    for Nx, Ny, Nz, off_lines, large_fp32, blocks in lines["biglin"]:
+        fits = blocks * 256 <= limit
+        assert off_lines == int(fits) and large_fp32 == int(fits), (Nx, Ny, Nz)
