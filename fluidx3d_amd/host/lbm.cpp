@@ -331,24 +331,25 @@ void LBM::backward() {
 }
 #endif
 
+// LBM status report in the reference's format (src/lbm.cpp:1047-1072), written to
+// <path>status-<t as 9 digits>.txt (default path: the executable's export/ folder)
 void LBM::write_status(const string& path) {
-	const string file = path == "" ? "status.txt" : path;
-	FILE* f = std::fopen(file.c_str(), "w");
-	if(!f) print_error("cannot open \"" + file + "\" for writing");
 	string s;
 	s += "Grid Resolution = " + to_string(Nx) + " x " + to_string(Ny) + " x " + to_string(Nz) + " = " + to_string(get_N()) + "\n";
-	s += "Grid Domains = 1 x 1 x 1 = 1\n";
+	s += "Grid Domains = " + to_string(Dx) + " x " + to_string(Dy) + " x " + to_string(Dz) + " = " + to_string(get_D()) + "\n";
 	s += "LBM Type = D3Q" + to_string(get_velocity_set()) + " " + info.collision + "\n";
-	s += "Memory Usage = CPU " + to_string(info.cpu_mem_required) + " MB, GPU 1x " + to_string(info.gpu_mem_required) + " MB\n";
-	s += "Time Steps = " + to_string(t) + "\n";
-	s += "Runtime = " + to_string(info.runtime_lbm, 3u) + " s (LBM)\n";
-	s += "Average MLUPs/s = " + to_string((ulong)(info.runtime_lbm > 0.0 ? 1E-6 * (double)get_N() * (double)t / info.runtime_lbm : 0.0)) + "\n";
-	s += "Kinematic Viscosity = " + to_string(nu) + "\n";
+	s += "Memory Usage = CPU " + to_string(info.cpu_mem_required) + " MB, GPU " + to_string(get_D()) + "x " + to_string(info.gpu_mem_required) + " MB\n";
+	s += "Maximum Allocation Size = " + to_string((uint)(get_N() / (ulong)get_D() * (ulong)(get_velocity_set() * sizeof(fpxx)) / 1048576ull)) + " MB\n";
+	s += "Time Steps = " + to_string(get_t()) + " / " + (info.steps == max_ulong ? string("infinite") : to_string(info.steps)) + "\n";
+	s += "Runtime = " + print_time(info.runtime_total) + " (total) = " + print_time(info.runtime_lbm) + " (LBM) + " + print_time(info.runtime_total - info.runtime_lbm) + " (rendering and data evaluation)\n";
+	s += "Average MLUPs/s = " + to_string(to_uint(1E-6 * (double)get_N() * (double)get_t() / info.runtime_lbm)) + "\n";
+	s += "Kinematic Viscosity = " + to_string(get_nu()) + "\n";
 	s += "Relaxation Time = " + to_string(get_tau()) + "\n";
 	s += "Maximum Reynolds Number = " + to_string(get_Re_max()) + "\n";
-	s += "Volume Force = " + to_string(fx) + ", " + to_string(fy) + ", " + to_string(fz) + "\n";
-	std::fwrite(s.data(), 1, s.size(), f);
-	std::fclose(f);
+#ifdef VOLUME_FORCE
+	s += "Volume Force = (" + to_string(get_fx()) + ", " + to_string(get_fy()) + ", " + to_string(get_fz()) + ")\n";
+#endif
+	write_file(default_filename(path, "status", ".txt", get_t()), s);
 }
 
 // every member of the two field types, including those no scene calls

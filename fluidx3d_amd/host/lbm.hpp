@@ -251,10 +251,10 @@ template<typename T> void Memory_Container<T>::write_to_device_3d(const ulong x0
 template<typename T> void Memory_Container<T>::write_to_device() {
 	fx_check(fx_buf_write(lbm->ctx, device, 0ull, capacity(), host.data(), 1), "writing " + name);
 }
+// `path` is a prefix, by default the executable's export/ folder: the file is
+// <path><name>-<t as 9 digits>.vtk (default_filename, src/lbm.hpp:398-404)
 template<typename T> void Memory_Container<T>::write_host_to_vtk(const string& path, const bool convert_to_si_units) {
-	const string digits = std::to_string(lbm->get_t());
-	const string stamp = string(9u - std::min<size_t>(9u, digits.size()), '0') + digits;
-	write_vtk(path == "" ? name + "-" + stamp + ".vtk" : path, convert_to_si_units);
+	write_vtk(default_filename(path, name, ".vtk", lbm->get_t()), convert_to_si_units);
 }
 template<typename T> void Memory_Container<T>::write_device_to_vtk(const string& path, const bool convert_to_si_units) {
 	read_from_device();
@@ -278,6 +278,7 @@ template<typename T> void Memory_Container<T>::write_vtk(const string& path, con
 	       << "ORIGIN " << to_string(spacing * (0.5f - 0.5f * (float)nx)) << " " << to_string(spacing * (0.5f - 0.5f * (float)ny)) << " " << to_string(spacing * (0.5f - 0.5f * (float)nz)) << "\n"
 	       << "SPACING " << to_string(spacing) << " " << to_string(spacing) << " " << to_string(spacing) << "\n"
 	       << "POINT_DATA " << N << "\nSCALARS data " << type << " " << d << "\nLOOKUP_TABLE default\n";
+	create_folder(path);
 	FILE* file = std::fopen(path.c_str(), "wb");
 	if(!file) print_error("cannot open \"" + path + "\" for writing");
 	const string h = header.str();

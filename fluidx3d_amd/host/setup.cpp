@@ -50,7 +50,7 @@ void main_setup() {
 	std::snprintf(line, sizeof(line), "t=%llu centre u.x=%.9e kinetic energy=%.12e mass drift=%.3e", (unsigned long long)lbm.get_t(), (double)lbm.u.x[lbm.index(Nx / 2u, Ny / 2u, Nz / 2u)], ke, (lbm.total_mass() - m0) / m0);
 	println(line);
 #ifdef FX_VTK
-	lbm.u.write_host_to_vtk("tgv_u.vtk", false);
+	lbm.u.write_host_to_vtk("export/", false); // export/u-<t>.vtk
 #endif
 }
 
@@ -91,6 +91,59 @@ void main_setup() {
 #endif
 }
 
+#elif defined(SCENE_STOKES)
+// Stokes drag validation, the reference's scene (src/setup.cpp:148-187): a sphere of
+// radius 32 in a 256^3 box at Re = 0.01, nu = 1; all six faces are TYPE_E cells preset to
+// the Stokes solution u_Stokes/rho_Stokes with far-field velocity (-u, 0, 0), as is the
+// initial fluid; every 100 steps the FP64 object force of the TYPE_S|TYPE_X sphere is
+// compared with F_Stokes = 6 pi rho nu R u until the error has converged.  Needs
+// FORCE_FIELD and EQUILIBRIUM_BOUNDARIES.
+void main_setup() {
+	const ulong dt = 100ull; // check error every dt time steps
+	const float R = 32.0f; // sphere radius
+	const float Re = 0.01f; // Reynolds number
+	const float nu = 1.0f; // kinematic shear viscosity
+	const float rho = 1.0f; // density
+	const uint L = to_uint(8.0f*R); // simulation box size
+	const float u = units.u_from_Re(Re, 2.0f*R, nu); // velocity
+	LBM lbm(L, L, L, nu); // flow driven by equilibrium boundaries
+	const uint Nx=lbm.get_Nx(), Ny=lbm.get_Ny(), Nz=lbm.get_Nz(); parallel_for(lbm.get_N(), [&](ulong n) { uint x=0u, y=0u, z=0u; lbm.coordinates(n, x, y, z);
+		if(x==0u||x==Nx-1u||y==0u||y==Ny-1u||z==0u||z==Nz-1u) lbm.flags[n] = TYPE_E;
+		if(sphere(x, y, z, lbm.center(), R)) {
+			lbm.flags[n] = TYPE_S|TYPE_X; // flag boundary cells for force summation additionally with TYPE_X
+		} else {
+			lbm.rho[n] = units.rho_Stokes(lbm.position(x, y, z), float3(-u, 0.0f, 0.0f), R, rho, nu);
+			const float3 un = units.u_Stokes(lbm.position(x, y, z), float3(-u, 0.0f, 0.0f), R);
+			lbm.u.x[n] = un.x;
+			lbm.u.y[n] = un.y;
+			lbm.u.z[n] = un.z;
+		}
+	});
+	double E1=1000.0, E2=1000.0;
+	while(true) { // main simulation loop
+		lbm.run(dt);
+		const float3 force = lbm.object_force(TYPE_S|TYPE_X);
+		const double F_theo = units.F_Stokes(rho, u, nu, R);
+		const double F_sim = (double)length(force);
+		const double E0 = fabs(F_sim-F_theo)/F_theo;
+		print_info(to_string(lbm.get_t())+", expected: "+to_string(F_theo, 6u)+", measured: "+to_string(F_sim, 6u)+", error = "+to_string((float)(100.0*E0), 1u)+"%");
+		if(converged(E2, E1, E0, 1E-4)) { // stop when error has sufficiently converged
+			print_info("Error converged after "+to_string(lbm.get_t())+" steps to "+to_string(100.0*E0, 1u)+"%");
+			wait();
+			break;
+		}
+#ifdef STOKES_MAX_STEPS
+		if(lbm.get_t() >= STOKES_MAX_STEPS) print_error("Stokes drag did not converge within " + to_string(lbm.get_t()) + " steps");
+#endif
+		E2 = E1;
+		E1 = E0;
+	}
+	const float3 F = lbm.object_force(TYPE_S|TYPE_X); // no step since the last check: the same force
+	char line[256];
+	std::snprintf(line, sizeof(line), "t=%llu F=(%.9e, %.9e, %.9e) F_Stokes=%.9e", (unsigned long long)lbm.get_t(), (double)F.x, (double)F.y, (double)F.z, (double)units.F_Stokes(rho, u, nu, R));
+	println(line);
+}
+
 #elif defined(SCENE_CAVITY)
 #ifndef CAVITY_L
 #define CAVITY_L 128u
@@ -115,7 +168,7 @@ void main_setup() {
 	std::snprintf(line, sizeof(line), "t=%llu centre u=(%.9e, %.9e, %.9e)", (unsigned long long)lbm.get_t(), (double)lbm.u.x[lbm.index(Nx/2u, Ny/2u, Nz/2u)], (double)lbm.u.y[lbm.index(Nx/2u, Ny/2u, Nz/2u)], (double)lbm.u.z[lbm.index(Nx/2u, Ny/2u, Nz/2u)]);
 	println(line);
 #ifdef FX_VTK
-	lbm.u.write_host_to_vtk("cavity_u.vtk", false);
+	lbm.u.write_host_to_vtk("export/", false);
 #endif
 }
 
@@ -145,7 +198,7 @@ void main_setup() {
 	std::snprintf(line, sizeof(line), "J=%.12e |dJ/dporosity|=%.6e forward %.1f ms backward %.1f ms", J, std::sqrt(norm), (double)lbm.forward_ms, (double)lbm.backward_ms);
 	println(line);
 #ifdef FX_VTK
-	lbm.porosity_gradient.write_host_to_vtk("porosity_gradient.vtk", false);
+	lbm.porosity_gradient.write_host_to_vtk("export/", false);
 #endif
 }
 

@@ -54,7 +54,7 @@ def test_cpp_taylor_green_matches_python_path(tmp_path):
     assert r.returncode == 0, r.stderr
     m = re.search(r"mass drift=([-0-9.e+]+)", r.stdout)
     assert m and abs(float(m.group(1))) < 1e-6
-    dims, u_cpp = read_vtk(os.path.join(tmp_path, "tgv_u.vtk"))
+    dims, u_cpp = read_vtk(os.path.join(tmp_path, "export", "u-000001000.vtk"))
     assert dims == [64, 64, 64]
     N = 64 ** 3
     lbm = fx.LBM(64, 64, 64, 1.0 / 6.0)
@@ -127,7 +127,7 @@ def test_cpp_porous_channel_backward_matches_python_path(tmp_path):
     assert m, r.stdout
     J_cpp = float(m.group(1))
     print("\n" + m.group(0))
-    dims, g_cpp = read_vtk(os.path.join(tmp_path, "porosity_gradient.vtk"))
+    dims, g_cpp = read_vtk(os.path.join(tmp_path, "export", "porosity_gradient-000002000.vtk"))
     Nx, Ny, Nz = 128, 64, 64
     assert dims == [Nx, Ny, Nz]
     flags, alpha = lcg_channel_porosity(Nx, Ny, Nz)
@@ -153,7 +153,7 @@ def test_cpp_lid_driven_cavity_matches_python_path(tmp_path):
     import fluidx3d_amd as fx
     r = subprocess.run([binary("cavity48")], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
-    dims, u_cpp = read_vtk(os.path.join(tmp_path, "cavity_u.vtk"))
+    dims, u_cpp = read_vtk(os.path.join(tmp_path, "export", "u-000001000.vtk"))
     L = 48
     assert dims == [L, L, L]
     nu = float(np.float32(np.float32(L - 2) * np.float32(0.1)) / np.float32(1000.0))  # units.nu_from_Re in float
@@ -186,3 +186,25 @@ def test_cpp_box_transfers_move_the_reference_elements(tmp_path):
     assert "box transfers: 30 cases, 0 wrong elements" in r.stdout, r.stdout
     # LBM::index(float3) / coordinates(float3) / mirror_position (src/lbm.hpp:484-515)
     assert "positions: 0 wrong cells" in r.stdout, r.stdout
+
+
+@pytest.mark.gpu
+def test_cpp_stokes_drag_scene(tmp_path):
+    """The reference's Stokes drag validation scene (src/setup.cpp:148-187), compiled against
+    this build's LBM API with FORCE_FIELD and EQUILIBRIUM_BOUNDARIES: sphere R = 32 in a
+    256^3 box at Re = 0.01, all faces TYPE_E preset to the Stokes solution; it runs until the
+    drag error has converged (slope and curvature below 1e-4 of the error per 100 steps).
+    The converged drag of the FP64 object sum must match F_Stokes = 6 pi rho nu R u within
+    the bound below, point against the far-field velocity (-u, 0, 0) and have no lateral
+    component (the scene is mirror symmetric in y and z)."""
+    r = subprocess.run([binary("stokes")], cwd=tmp_path, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    m = re.search(r"t=(\d+) F=\(([-0-9.e+]+), ([-0-9.e+]+), ([-0-9.e+]+)\) F_Stokes=([-0-9.e+]+)", r.stdout)
+    assert m, r.stdout[-2000:]
+    t = int(m.group(1))
+    Fx, Fy, Fz, F_theo = (float(v) for v in m.groups()[1:])
+    err = abs(np.sqrt(Fx * Fx + Fy * Fy + Fz * Fz) - F_theo) / F_theo
+    print(f"\nStokes drag: t={t}, F=({Fx:.6e}, {Fy:.3e}, {Fz:.3e}), F_Stokes={F_theo:.6e}, error {100 * err:.2f}%")
+    assert abs(F_theo - 6 * np.pi * 1.0 * 1.0 * 32.0 * (0.01 * 1.0 / 64.0)) < 1e-6 * F_theo
+    assert Fx < 0 and abs(Fy) < 1e-3 * abs(Fx) and abs(Fz) < 1e-3 * abs(Fx)
+    assert err < 0.1
