@@ -121,6 +121,52 @@ def test_stream_collide_matches_oracle(oracle, q, collision, storage, ext, grid)
         lbm.close()
 
 
+CLAMP_SETS = {
+    "plain": (),
+    "volume_force": ("VOLUME_FORCE",),
+    "equilibrium": ("EQUILIBRIUM_BOUNDARIES",),  # TYPE_E presets above def_c, clamped in the emission
+    "forces": ("VOLUME_FORCE", "FORCE_FIELD", "EQUILIBRIUM_BOUNDARIES", "UPDATE_FIELDS"),
+}
+
+
+@pytest.mark.parametrize("grid", ["64x6x5", "20x12x10"])
+@pytest.mark.parametrize("q,collision,storage,ext", list(itertools.product((19, 27), ("SRT", "TRT"), ("FP32", "FP16S", "FP16C"), CLAMP_SETS)))
+def test_velocity_clamp_matches_oracle(oracle, q, collision, storage, ext, grid):
+    """A coherent fast flow, u ~ (0.6, -0.58, 0) +- 0.1 (TYPE_E presets too), with a strong
+    body force where enabled, so that hundreds of cells per step have velocity components
+    beyond def_c = 0.57735027 and take the clamp (src/kernel.cpp:1553-1560, 1853-1855):
+    6 steps, every DDF compared after every step, then rho/u after update_fields."""
+    from tests.test_oracle_clamp import fast_scene
+    dims = {"64x6x5": (64, 6, 5), "20x12x10": (20, 12, 10)}[grid]
+    Nx, Ny, Nz = dims
+    N = Nx * Ny * Nz
+    names = CLAMP_SETS[ext]
+    rho, u, flags = fast_scene(Nx, Ny, Nz, seed=q + len(names), equilibrium_faces="EQUILIBRIUM_BOUNDARIES" in names)
+    rng = np.random.default_rng(3)
+    F = (1e-3 * rng.uniform(-1, 1, 3 * N)).astype(np.float32) if "FORCE_FIELD" in names else None
+    f = (0.01, -0.01, 0.005) if "VOLUME_FORCE" in names else (0.0, 0.0, 0.0)
+    lbm, ref = setup_pair(oracle, Nx, Ny, Nz, 0.1, q, collision, storage, names, rho, u, flags, F, None, f)
+    try:
+        lbm.initialize()
+        ref.initialize()
+        compare_state(lbm, ref, oracle, storage, "after initialize")
+        clamped = 0
+        for s in range(6):
+            lbm.run(1)
+            ref.step(1)
+            compare_state(lbm, ref, oracle, storage, f"after step {s + 1}")
+            if "UPDATE_FIELDS" not in names:
+                ref.update_fields()
+            clamped += int((np.abs(ref.u) == np.float32(0.57735027)).sum())
+        assert clamped > 100, clamped
+        lbm.u.read_from_device()
+        lbm.rho.read_from_device()
+        assert np.array_equal(lbm.rho.host, ref.rho), "rho differs after update_fields"
+        assert np.array_equal(lbm.u.host, ref.u), "u differs after update_fields"
+    finally:
+        lbm.close()
+
+
 @pytest.mark.parametrize("storage", ["FP32", "FP16S", "FP16C"])
 def test_tgv_64_1000_steps(oracle, storage):
     """Config C1 shape: D3Q19 SRT 64^3 Taylor-Green, nu=1/6, u0=0.05, 1000 steps.
