@@ -126,7 +126,6 @@ void LBM::construct(const uint nx, const uint ny, const uint nz, const uint dx, 
 	(void)particles_rho;
 	if((ulong)nx * (ulong)ny * (ulong)nz == 0ull) print_error("Grid point number is 0: " + to_string(nx) + "x" + to_string(ny) + "x" + to_string(nz) + " = 0.");
 	if(dx * dy * dz == 0u) print_error("You specified 0 LBM grid domains (" + to_string(dx) + "x" + to_string(dy) + "x" + to_string(dz) + ").");
-	if(dx * dy * dz != 1u) print_error("This build runs one lattice on one GPU; multi-domain decomposition (" + to_string(dx) + "x" + to_string(dy) + "x" + to_string(dz) + ") is not available.");
 	if(viscosity == 0.0f) print_error("Viscosity cannot be 0. Change it in setup.cpp.");
 	if(viscosity < 0.0f) print_error("Viscosity cannot be negative. Remove the \"-\" in setup.cpp.");
 #if !defined(VOLUME_FORCE) && !defined(POROSITY)
@@ -135,15 +134,38 @@ void LBM::construct(const uint nx, const uint ny, const uint nz, const uint dx, 
 	if(sigma != 0.0f) print_error("Surface tension is set in LBM constructor in main_setup(), but SURFACE is not available in this build.");
 	if(alpha != 0.0f || beta != 0.0f) print_error("Thermal diffusion/expansion coefficients are set in LBM constructor in main_setup(), but TEMPERATURE is not available in this build.");
 	if(particles_N > 0u) print_error("The number of particles is set to " + to_string(particles_N) + ">0, but PARTICLES is not available in this build.");
-	Nx = nx; Ny = ny; Nz = nz; Dx = dx; Dy = dy; Dz = dz;
+	// resolution divisible by the domains (src/lbm.cpp:721-725)
+	const uint ndx = (nx / dx) * dx, ndy = (ny / dy) * dy, ndz = (nz / dz) * dz;
+	if(ndx != nx || ndy != ny || ndz != nz) print_warning("LBM grid (" + to_string(nx) + "x" + to_string(ny) + "x" + to_string(nz) + ") is not equally divisible in domains (" + to_string(dx) + "x" + to_string(dy) + "x" + to_string(dz) + "). Changing resolution to (" + to_string(ndx) + "x" + to_string(ndy) + "x" + to_string(ndz) + ").");
+	if((ulong)ndx * (ulong)ndy * (ulong)ndz == 0ull) print_error("Grid point number is 0 after dividing into domains.");
+	Nx = ndx; Ny = ndy; Nz = ndz; Dx = dx; Dy = dy; Dz = dz;
 	nu = viscosity; fx = f_x; fy = f_y; fz = f_z;
-	cfg = lbm_config(Nx, Ny, Nz, nu);
 	int devices = 0;
 	fx_check(fx_device_count(&devices), "device discovery");
 	if(devices == 0) print_error("There are no GPU devices available.");
-	fx_check(fx_ctx_create(0, &cfg, &ctx), "creating the LBM context");
+	// domains d = x + (y + z*Dy)*Dx of Nx/Dx (+2 halo cells along a split axis), all on
+	// device 0 with a context each (src/lbm.cpp:727-733)
+	const uint hx = Dx > 1u, hy = Dy > 1u, hz = Dz > 1u, D = get_D();
+	domain.assign(D, LBM_Domain());
+	for(uint d = 0u; d < D; d++) {
+		LBM_Domain& dom = domain[d];
+		const uint px = d % Dx, py = (d / Dx) % Dy, pz = d / (Dx * Dy);
+		dom.Nx = Nx / Dx + 2u * hx; dom.Ny = Ny / Dy + 2u * hy; dom.Nz = Nz / Dz + 2u * hz;
+		dom.ox = (int)(px * (Nx / Dx)) - (int)hx; dom.oy = (int)(py * (Ny / Dy)) - (int)hy; dom.oz = (int)(pz * (Nz / Dz)) - (int)hz;
+		dom.cfg = lbm_config(dom.Nx, dom.Ny, dom.Nz, nu);
+		fx_check(fx_ctx_create(0, &dom.cfg, &dom.ctx), "creating the LBM context of domain " + to_string(d));
+		fx_check(fx_buf_alloc(dom.ctx, dom.get_N() * (ulong)velocity_set * sizeof(fpxx), &dom.fi), "allocating DDFs");
+		const uint split[3] = {Dx, Dy, Dz};
+		for(uint axis = 0u; axis < 3u; axis++) {
+			if(split[axis] == 1u) continue;
+			ulong bytes = 0ull;
+			for(uint field = FX_FIELD_FI; field <= FX_FIELD_F; field++) bytes = max(bytes, (ulong)fx_transfer_bytes(&dom.cfg, field, axis));
+			fx_check(fx_buf_alloc(dom.ctx, bytes, &dom.transfer_p[axis]), "allocating transfer buffers");
+			fx_check(fx_buf_alloc(dom.ctx, bytes, &dom.transfer_m[axis]), "allocating transfer buffers");
+		}
+	}
+	ctx = domain[0].ctx; fi = domain[0].fi; cfg = domain[0].cfg;
 	const ulong N = get_N();
-	fx_check(fx_buf_alloc(ctx, N * (ulong)velocity_set * sizeof(fpxx), &fi), "allocating DDFs");
 	rho.allocate(this, N, 1u, "rho", 1.0f);
 	u.allocate(this, N, 3u, "u", 0.0f);
 	flags.allocate(this, N, 1u, "flags", (uchar)0u);
@@ -167,8 +189,74 @@ LBM::~LBM() {
 	porosity_gradient.release();
 	if(adjoint) fx_adjoint_destroy(adjoint);
 #endif
-	if(ctx) fx_ctx_destroy(ctx); // frees fi with the context
+	for(LBM_Domain& dom : domain) if(dom.ctx) fx_ctx_destroy(dom.ctx); // frees fi and the transfer buffers with the context
+	domain.clear();
 	ctx = nullptr;
+}
+
+void LBM::require_single_domain(const string& what) const {
+	if(get_D() != 1u) print_error(what + " addresses one domain's buffers and is not available on a lattice split into " + to_string(Dx) + "x" + to_string(Dy) + "x" + to_string(Dz) + " domains.");
+}
+void LBM::sync_domains() {
+	for(LBM_Domain& dom : domain) fx_check(fx_sync(dom.ctx), "synchronising the domains");
+}
+// domain next to d along axis in direction step (+1/-1), periodic
+uint LBM::neighbour(const uint d, const uint axis, const int step) const {
+	uint c[3] = {d % Dx, (d / Dx) % Dy, d / (Dx * Dy)};
+	const uint n[3] = {Dx, Dy, Dz};
+	c[axis] = (uint)(((int)c[axis] + step + (int)n[axis]) % (int)n[axis]);
+	return c[0] + (c[1] + c[2] * Dy) * Dx;
+}
+// Per split axis: every domain extracts the values crossing its two faces into its
+// buffers (kernel on its own stream), all finish, then every domain inserts the buffers
+// of its neighbours (device to device: all domains share the device), all finish.
+void LBM::communicate(const uint field) {
+	const uint split[3] = {Dx, Dy, Dz};
+	for(uint axis = 0u; axis < 3u; axis++) {
+		if(split[axis] == 1u) continue;
+		for(uint d = 0u; d < get_D(); d++) {
+			LBM_Domain& dom = domain[d];
+			fx_check(fx_lbm_transfer_extract(dom.ctx, field, axis, t, dom.transfer_p[axis], dom.transfer_m[axis], dom.fi, (const float*)rho.device_buffer(d), (const float*)u.device_buffer(d), (const uint8_t*)flags.device_buffer(d),
+#ifdef FORCE_FIELD
+			                                 (const float*)F.device_buffer(d)),
+#else
+			                                 nullptr),
+#endif
+			         "halo extract");
+		}
+		sync_domains();
+		for(uint d = 0u; d < get_D(); d++) {
+			LBM_Domain& dom = domain[d];
+			const void* from_p = domain[neighbour(d, axis, +1)].transfer_m[axis]; // the + neighbour's - side
+			const void* from_m = domain[neighbour(d, axis, -1)].transfer_p[axis]; // the - neighbour's + side
+			fx_check(fx_lbm_transfer_insert(dom.ctx, field, axis, t, from_p, from_m, dom.fi, (float*)rho.device_buffer(d), (float*)u.device_buffer(d), (uint8_t*)flags.device_buffer(d),
+#ifdef FORCE_FIELD
+			                                (float*)F.device_buffer(d)),
+#else
+			                                nullptr),
+#endif
+			         "halo insert");
+		}
+		sync_domains();
+	}
+}
+void LBM::update_fields_on_device() {
+	if(!initialized || t_last_update_fields == t) return;
+	for(uint d = 0u; d < get_D(); d++) {
+		fx_check(fx_lbm_update_fields(domain[d].ctx, domain[d].fi, (float*)rho.device_buffer(d), (float*)u.device_buffer(d), (const uint8_t*)flags.device_buffer(d), t, fx, fy, fz,
+#ifdef FORCE_FIELD
+		                              (const float*)F.device_buffer(d),
+#else
+		                              nullptr,
+#endif
+#ifdef POROSITY
+		                              (const float*)porosity.device_buffer(d)),
+#else
+		                              nullptr),
+#endif
+		         "update_fields");
+	}
+	t_last_update_fields = t;
 }
 
 // flags used vs extensions enabled, src/lbm.cpp:840-879
@@ -208,8 +296,20 @@ void LBM::initialize() {
 #ifdef POROSITY
 	porosity.write_to_device();
 #endif
-	fx_check(fx_lbm_initialize(ctx, fi, (const float*)rho.device_buffer(), (float*)u.device_buffer(), (uint8_t*)flags.device_buffer()), "initialize");
-	fx_check(fx_sync(ctx), "initialize");
+	if(get_D() == 1u) {
+		fx_check(fx_lbm_initialize(ctx, fi, (const float*)rho.device_buffer(), (float*)u.device_buffer(), (uint8_t*)flags.device_buffer()), "initialize");
+		fx_check(fx_sync(ctx), "initialize");
+	} else { // halos as the reference fills them (src/lbm.cpp:884-920); t odd for the transfers
+#ifdef FORCE_FIELD
+		communicate(FX_FIELD_F);
+#endif
+		t = 1ull;
+		communicate(FX_FIELD_RHO_U_FLAGS);
+		for(uint d = 0u; d < get_D(); d++) fx_check(fx_lbm_initialize(domain[d].ctx, domain[d].fi, (const float*)rho.device_buffer(d), (float*)u.device_buffer(d), (uint8_t*)flags.device_buffer(d)), "initialize");
+		communicate(FX_FIELD_RHO_U_FLAGS);
+		communicate(FX_FIELD_FI);
+		sync_domains();
+	}
 	t = 0ull;
 	t_last_update_fields = max_ulong;
 	t_last_force_field = max_ulong;
@@ -220,8 +320,9 @@ void LBM::initialize() {
 // mark/unmark cells next to moving solid cells with TYPE_MS after the host changed wall
 // velocities (u written to the device), src/lbm.cpp:1019-1027
 void LBM::update_moving_boundaries() {
-	fx_check(fx_lbm_update_moving_boundaries(ctx, (const float*)u.device_buffer(), (uint8_t*)flags.device_buffer()), "update_moving_boundaries");
-	fx_check(fx_sync(ctx), "update_moving_boundaries");
+	for(uint d = 0u; d < get_D(); d++) fx_check(fx_lbm_update_moving_boundaries(domain[d].ctx, (const float*)u.device_buffer(d), (uint8_t*)flags.device_buffer(d)), "update_moving_boundaries");
+	if(get_D() > 1u) communicate(FX_FIELD_FLAGS);
+	sync_domains();
 }
 #endif
 
@@ -246,15 +347,40 @@ void LBM::run(const ulong steps, const ulong total_steps) {
 		nullptr;
 #endif
 	ulong done = 0ull;
-	while(done < steps) {
-		const ulong batch = std::min<ulong>(steps - done, 1000ull);
-		float ms = 0.0f;
-		fx_check(fx_lbm_run(ctx, fi, (float*)rho.device_buffer(), (float*)u.device_buffer(), (const uint8_t*)flags.device_buffer(), t, batch, fx, fy, fz, Fp, ap, &ms), "stream_collide");
-		const double dt = 1E-3 * (double)ms / (double)batch;
-		for(ulong s = 0ull; s < batch; s++) info.update(dt);
-		t += batch;
-		done += batch;
+	if(get_D() == 1u) {
+		while(done < steps) {
+			const ulong batch = std::min<ulong>(steps - done, 1000ull);
+			float ms = 0.0f;
+			fx_check(fx_lbm_run(ctx, fi, (float*)rho.device_buffer(), (float*)u.device_buffer(), (const uint8_t*)flags.device_buffer(), t, batch, fx, fy, fz, Fp, ap, &ms), "stream_collide");
+			const double dt = 1E-3 * (double)ms / (double)batch;
+			for(ulong s = 0ull; s < batch; s++) info.update(dt);
+			t += batch;
+			done += batch;
+		}
+	} else { // do_time_step (src/lbm.cpp:924-953): every domain, then the DDF halos
+		Clock clock;
+		for(; done < steps; done++) {
+			clock.start();
+			for(uint d = 0u; d < get_D(); d++) {
+				fx_check(fx_lbm_stream_collide(domain[d].ctx, domain[d].fi, (float*)rho.device_buffer(d), (float*)u.device_buffer(d), (const uint8_t*)flags.device_buffer(d), t, fx, fy, fz,
+#ifdef FORCE_FIELD
+				                               (const float*)F.device_buffer(d),
+#else
+				                               nullptr,
+#endif
+#ifdef POROSITY
+				                               (const float*)porosity.device_buffer(d)),
+#else
+				                               nullptr),
+#endif
+				         "stream_collide");
+			}
+			communicate(FX_FIELD_FI);
+			info.update(clock.stop());
+			t++;
+		}
 	}
+	(void)Fp; (void)ap;
 #ifdef UPDATE_FIELDS
 	t_last_update_fields = t;
 #endif
@@ -262,47 +388,111 @@ void LBM::run(const ulong steps, const ulong total_steps) {
 
 void LBM::update_fields() {
 	if(!initialized) return;
-	rho.read_from_device(); // runs update_fields if stale
-	u.read_from_device();
+	update_fields_on_device();
+	sync_domains();
 }
 
 void LBM::reset() { initialized = false; }
 
+// DDF codes as the kernels store them (src/lbm.cpp:410-414, src/kernel.cpp:848-859), for
+// the host-side sum over the interior cells of split lattices
+static inline float ddf_value(const float v) { return v; }
+static inline float ddf_value(const ushort c) {
+#if defined(FP16S) // IEEE binary16 times 2^-15
+	const uint sign = (uint)(c & 0x8000u) << 16, e = (c >> 10) & 0x1Fu, m = c & 0x3FFu;
+	float v;
+	if(e == 0u) v = std::ldexp((float)m, -24); // subnormal
+	else if(e == 31u) v = m ? NAN : INFINITY;
+	else v = as_float(((e + 112u) << 23) | (m << 13));
+	return std::ldexp(as_float(as_uint(v) | sign), -15);
+#else // 1-4-11 with bias 15: exponent and mantissa at the binary32 positions, times 2^112
+	return std::ldexp(as_float(((uint)(c & 0x8000u) << 16) | ((uint)(c & 0x7FFFu) << 12)), 112);
+#endif
+}
 double LBM::total_mass() {
 	double m = 0.0;
-	fx_check(fx_lbm_mass_sum(ctx, fi, (const uint8_t*)flags.device_buffer(), t, &m), "mass sum");
+	if(get_D() == 1u) {
+		fx_check(fx_lbm_mass_sum(ctx, fi, (const uint8_t*)flags.device_buffer(), t, &m), "mass sum");
+		return m;
+	}
+	// split lattice: 1 plus the DDFs in the own slots of every interior cell, as the
+	// undivided lattice's sum; the halo cells are copies and are left out
+	const uint hx = Dx > 1u, hy = Dy > 1u, hz = Dz > 1u;
+	for(const LBM_Domain& dom : domain) {
+		const ulong NL = dom.get_N();
+		vector<fpxx> local(NL * (ulong)velocity_set);
+		fx_check(fx_buf_read(dom.ctx, dom.fi, 0ull, local.size() * sizeof(fpxx), local.data(), 1), "reading DDFs");
+		for(uint lz = hz; lz < dom.Nz - hz; lz++) for(uint ly = hy; ly < dom.Ny - hy; ly++) for(uint lx = hx; lx < dom.Nx - hx; lx++) {
+			const ulong l = (ulong)lx + ((ulong)ly + (ulong)lz * dom.Ny) * dom.Nx;
+			m += 1.0;
+			for(uint i = 0u; i < velocity_set; i++) m += (double)ddf_value(local[(ulong)i * NL + l]);
+		}
+	}
 	return m;
 }
 
 #ifdef FORCE_FIELD
 void LBM::update_force_field() {
 	if(t_last_force_field != t) {
-		fx_check(fx_lbm_update_force_field(ctx, fi, (const uint8_t*)flags.device_buffer(), t, (float*)F.device_buffer()), "update_force_field");
+		for(uint d = 0u; d < get_D(); d++) fx_check(fx_lbm_update_force_field(domain[d].ctx, domain[d].fi, (const uint8_t*)flags.device_buffer(d), t, (float*)F.device_buffer(d)), "update_force_field");
 		t_last_force_field = t;
 	}
-	fx_check(fx_sync(ctx), "update_force_field");
+	sync_domains();
 }
+// Object sums.  One domain: FP64 reductions on the device.  A split lattice: the interior
+// cells' F and flags come to the host and are summed there in FP64 in cell order, per-cell
+// terms in FP32 as on the device, so the result is the undivided lattice's to FP64 rounding.
 float3 LBM::object_center_of_mass(const uchar flag_marker) {
-	double s[4];
-	fx_check(fx_lbm_object_center_of_mass(ctx, (const uint8_t*)flags.device_buffer(), flag_marker, s), "object_center_of_mass");
+	double s[4] = {0.0, 0.0, 0.0, 0.0};
+	if(get_D() == 1u) {
+		fx_check(fx_lbm_object_center_of_mass(ctx, (const uint8_t*)flags.device_buffer(), flag_marker, s), "object_center_of_mass");
+	} else {
+		flags.read_from_device();
+		for(ulong n = 0ull; n < get_N(); n++) {
+			if(flags[n] != flag_marker) continue;
+			const float3 p = position(n);
+			s[0] += (double)p.x; s[1] += (double)p.y; s[2] += (double)p.z; s[3] += 1.0;
+		}
+	}
 	return float3((float)(s[0] / s[3]), (float)(s[1] / s[3]), (float)(s[2] / s[3]));
 }
 float3 LBM::object_force(const uchar flag_marker) {
 	update_force_field();
-	double s[3];
-	fx_check(fx_lbm_object_force(ctx, (const float*)F.device_buffer(), (const uint8_t*)flags.device_buffer(), flag_marker, s), "object_force");
+	double s[3] = {0.0, 0.0, 0.0};
+	if(get_D() == 1u) {
+		fx_check(fx_lbm_object_force(ctx, (const float*)F.device_buffer(), (const uint8_t*)flags.device_buffer(), flag_marker, s), "object_force");
+	} else {
+		F.read_from_device();
+		flags.read_from_device();
+		for(ulong n = 0ull; n < get_N(); n++) {
+			if(flags[n] != flag_marker) continue;
+			s[0] += (double)F.x[n]; s[1] += (double)F.y[n]; s[2] += (double)F.z[n];
+		}
+	}
 	return float3((float)s[0], (float)s[1], (float)s[2]);
 }
 float3 LBM::object_torque(const float3& rotation_center, const uchar flag_marker) {
 	update_force_field();
-	double s[3];
-	fx_check(fx_lbm_object_torque(ctx, (const float*)F.device_buffer(), (const uint8_t*)flags.device_buffer(), flag_marker, rotation_center.x, rotation_center.y, rotation_center.z, s), "object_torque");
+	double s[3] = {0.0, 0.0, 0.0};
+	if(get_D() == 1u) {
+		fx_check(fx_lbm_object_torque(ctx, (const float*)F.device_buffer(), (const uint8_t*)flags.device_buffer(), flag_marker, rotation_center.x, rotation_center.y, rotation_center.z, s), "object_torque");
+	} else {
+		F.read_from_device();
+		flags.read_from_device();
+		for(ulong n = 0ull; n < get_N(); n++) {
+			if(flags[n] != flag_marker) continue;
+			const float3 r = position(n) - rotation_center;
+			const float Fx = F.x[n], Fy = F.y[n], Fz = F.z[n];
+			s[0] += (double)(r.y * Fz - r.z * Fy); s[1] += (double)(r.z * Fx - r.x * Fz); s[2] += (double)(r.x * Fy - r.y * Fx);
+		}
+	}
 	return float3((float)s[0], (float)s[1], (float)s[2]);
 }
 #endif
 
 #ifdef POROSITY
 double LBM::forward(const ulong steps, const ulong window_begin, const uint plane_x, const ulong checkpoint_interval) {
+	require_single_domain("forward()/backward()");
 	if(adjoint) { fx_adjoint_destroy(adjoint); adjoint = nullptr; }
 	fx_adjoint_config c{steps, checkpoint_interval, window_begin, plane_x};
 	fx_check(fx_adjoint_create(ctx, &c, &adjoint), "adjoint workspace");

@@ -1,7 +1,8 @@
 #pragma once
 // The LBM host class with the reference's interface (src/lbm.hpp:205-611) on top of the
-// C ABI in include/fx_lbm.h.  One lattice on one GPU: the reference's multi-domain
-// constructor accepts only Dx=Dy=Dz=1 here.
+// C ABI in include/fx_lbm.h.  A lattice may be split into Dx*Dy*Dz domains with one halo
+// layer per split side, as in the reference (src/lbm.cpp:721-773); all domains run on
+// device 0, each with its own context (stream), and exchange halos device to device.
 #include "defines.hpp"
 #include "utilities.hpp"
 #include "units.hpp"
@@ -18,6 +19,22 @@ fx_lbm_config lbm_config(const uint Nx, const uint Ny, const uint Nz, const floa
 class LBM;
 void fx_check(const int status, const string& what); // nonzero status -> print_error, like the reference
 
+// One box of a lattice split into Dx*Dy*Dz domains (LBM_Domain, src/lbm.cpp:96-176,
+// 721-773): its own context (device, stream, kernel variant) and DDFs; Nx/Dx + 2 cells
+// along a split axis (one halo layer per side), Nx along an unsplit one; (ox, oy, oz) are
+// the global coordinates of local cell (0, 0, 0), -1 along a split axis.  The halo transfer
+// buffers hold one side's values of the largest field per split axis.
+struct LBM_Domain {
+	fx_ctx* ctx = nullptr;
+	void* fi = nullptr;
+	fx_lbm_config cfg{};
+	uint Nx = 1u, Ny = 1u, Nz = 1u;
+	int ox = 0, oy = 0, oz = 0;
+	void* transfer_p[3] = {nullptr, nullptr, nullptr};
+	void* transfer_m[3] = {nullptr, nullptr, nullptr};
+	ulong get_N() const { return (ulong)Nx * (ulong)Ny * (ulong)Nz; }
+};
+
 // Host buffer + device buffer of one field; the single-domain Memory_Container of the
 // reference (src/lbm.hpp:239-408).  Components are stored one after another (u.x[n] at
 // n, u.y[n] at N+n, u.z[n] at 2N+n).
@@ -27,9 +44,14 @@ template<typename T> class Memory_Container {
 	uint d = 1u;
 	string name;
 	vector<T> host;
-	void* device = nullptr;
+	vector<void*> devices; // one buffer per domain (local cells, halos included)
+	void* device = nullptr; // devices[0]
 	void write_vtk(const string& path, const bool convert_to_si_units);
 	void refresh(); // update_fields when rho/u on the device are stale
+	// global host array <-> one domain's local layout: every local cell (halos included,
+	// wrapped periodically) from the host, interior cells back to it
+	void gather(const LBM_Domain& dom, vector<T>& local) const;
+	void scatter(const LBM_Domain& dom, const vector<T>& local);
 	void box(const bool to_device, const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong z0, const ulong z1, const ulong Nx, const ulong Ny, const int dimension, const bool blocking);
 public:
 	class Pointer {
@@ -59,6 +81,7 @@ public:
 	ulong capacity() const { return N * (ulong)d * sizeof(T); }
 	T* data() { return host.data(); }
 	void* device_buffer() const { return device; }
+	void* device_buffer(const uint domain) const { return devices[domain]; }
 	void reset(const T value = (T)0) { std::fill(host.begin(), host.end(), value); }
 	void read_from_device();  // refreshes rho/u on the device first when stale (src/lbm.hpp:390-396)
 	void write_to_device();
@@ -84,15 +107,23 @@ class LBM {
 	bool initialized = false;
 	ulong t_last_update_fields = max_ulong;
 	ulong t_last_force_field = max_ulong;
-	fx_ctx* ctx = nullptr;
-	void* fi = nullptr; // DDFs, device only
-	fx_lbm_config cfg{};
+	fx_ctx* ctx = nullptr; // domain[0].ctx
+	void* fi = nullptr;    // domain[0].fi: DDFs, device only
+	fx_lbm_config cfg{};   // domain[0].cfg
 	fx_adjoint* adjoint = nullptr;
+	vector<LBM_Domain> domain; // d = x + (y + z*Dy)*Dx
 
 	void construct(const uint Nx, const uint Ny, const uint Nz, const uint Dx, const uint Dy, const uint Dz, const float nu, const float fx, const float fy, const float fz,
 	               const float sigma, const float alpha, const float beta, const uint particles_N, const float particles_rho);
 	void sanity_checks_initialization();
 	void initialize();
+	void update_fields_on_device(); // update_fields on every domain if stale
+	void sync_domains();
+	// halo exchange of one field after the extract kernels, axis by axis (x, y, z):
+	// LBM::communicate_field (src/lbm.cpp:1355-1383), device to device
+	void communicate(const uint field);
+	uint neighbour(const uint d, const uint axis, const int step) const;
+	void require_single_domain(const string& what) const;
 	template<typename T> friend class Memory_Container;
 
 public:
@@ -203,39 +234,70 @@ template<typename T> void Memory_Container<T>::allocate(LBM* l, const ulong n, c
 	lbm = l; N = n; d = dims; name = field_name;
 	if(N * (ulong)d == 0ull) print_error("Memory size must be larger than 0.");
 	host.assign(N * (ulong)d, fill);
-	fx_check(fx_buf_alloc(lbm->ctx, N * (ulong)d * sizeof(T), &device), "allocating " + name);
+	devices.assign(lbm->domain.size(), nullptr);
+	for(size_t k = 0u; k < lbm->domain.size(); k++)
+		fx_check(fx_buf_alloc(lbm->domain[k].ctx, lbm->domain[k].get_N() * (ulong)d * sizeof(T), &devices[k]), "allocating " + name);
+	device = devices[0];
 	x = Pointer(this, 0u);
 	if(d > 1u) y = Pointer(this, 1u);
 	if(d > 2u) z = Pointer(this, 2u);
 }
 template<typename T> void Memory_Container<T>::release() {
-	if(device && lbm && lbm->ctx) fx_buf_free(lbm->ctx, device);
+	if(lbm) for(size_t k = 0u; k < devices.size(); k++) if(devices[k] && lbm->domain[k].ctx) fx_buf_free(lbm->domain[k].ctx, devices[k]);
+	devices.clear();
 	device = nullptr;
+}
+template<typename T> void Memory_Container<T>::gather(const LBM_Domain& dom, vector<T>& local) const {
+	const ulong Nx = lbm->Nx, Ny = lbm->Ny, Nz = lbm->Nz, NL = dom.get_N();
+	local.resize(NL * (ulong)d);
+	parallel_for((ulong)dom.Nz, [&](ulong lz) {
+		const ulong gz = (ulong)(((long long)lz + dom.oz + (long long)Nz) % (long long)Nz);
+		for(uint ly = 0u; ly < dom.Ny; ly++) {
+			const ulong gy = (ulong)(((long long)ly + dom.oy + (long long)Ny) % (long long)Ny);
+			for(uint lx = 0u; lx < dom.Nx; lx++) {
+				const ulong gx = (ulong)(((long long)lx + dom.ox + (long long)Nx) % (long long)Nx);
+				const ulong g = gx + (gy + gz * Ny) * Nx, l = (ulong)lx + ((ulong)ly + lz * (ulong)dom.Ny) * (ulong)dom.Nx;
+				for(uint c = 0u; c < d; c++) local[l + (ulong)c * NL] = host[g + (ulong)c * N];
+			}
+		}
+	});
+}
+template<typename T> void Memory_Container<T>::scatter(const LBM_Domain& dom, const vector<T>& local) {
+	const ulong Nx = lbm->Nx, Ny = lbm->Ny, NL = dom.get_N();
+	const uint hx = lbm->Dx > 1u, hy = lbm->Dy > 1u, hz = lbm->Dz > 1u;
+	parallel_for((ulong)(dom.Nz - 2u * hz), [&](ulong iz) {
+		const ulong lz = iz + hz, gz = (ulong)((long long)lz + dom.oz);
+		for(uint ly = hy; ly < dom.Ny - hy; ly++) {
+			const ulong gy = (ulong)((long long)ly + dom.oy);
+			for(uint lx = hx; lx < dom.Nx - hx; lx++) {
+				const ulong gx = (ulong)((long long)lx + dom.ox);
+				const ulong g = gx + (gy + gz * Ny) * Nx, l = (ulong)lx + ((ulong)ly + lz * (ulong)dom.Ny) * (ulong)dom.Nx;
+				for(uint c = 0u; c < d; c++) host[g + (ulong)c * N] = local[l + (ulong)c * NL];
+			}
+		}
+	});
 }
 template<typename T> void Memory_Container<T>::read_from_device() {
 	refresh();
-	fx_check(fx_buf_read(lbm->ctx, device, 0ull, capacity(), host.data(), 1), "reading " + name);
+	if(lbm->domain.size() == 1u) {
+		fx_check(fx_buf_read(lbm->ctx, device, 0ull, capacity(), host.data(), 1), "reading " + name);
+		return;
+	}
+	vector<T> local;
+	for(size_t k = 0u; k < lbm->domain.size(); k++) {
+		const LBM_Domain& dom = lbm->domain[k];
+		local.resize(dom.get_N() * (ulong)d);
+		fx_check(fx_buf_read(dom.ctx, devices[k], 0ull, local.size() * sizeof(T), local.data(), 1), "reading " + name);
+		scatter(dom, local);
+	}
 }
 template<typename T> void Memory_Container<T>::refresh() {
 #ifndef UPDATE_FIELDS
-	if(lbm->initialized && lbm->t_last_update_fields != lbm->t) {
-		fx_check(fx_lbm_update_fields(lbm->ctx, lbm->fi, (float*)lbm->rho.device_buffer(), (float*)lbm->u.device_buffer(), (const uint8_t*)lbm->flags.device_buffer(), lbm->t, lbm->fx, lbm->fy, lbm->fz,
-#ifdef FORCE_FIELD
-		                              (const float*)lbm->F.device_buffer(),
-#else
-		                              nullptr,
-#endif
-#ifdef POROSITY
-		                              (const float*)lbm->porosity.device_buffer()),
-#else
-		                              nullptr),
-#endif
-		         "update_fields");
-		lbm->t_last_update_fields = lbm->t;
-	}
+	lbm->update_fields_on_device();
 #endif
 }
 template<typename T> void Memory_Container<T>::box(const bool to_device, const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong z0, const ulong z1, const ulong Nx, const ulong Ny, const int dimension, const bool blocking) {
+	lbm->require_single_domain("box transfers of " + name);
 	if(to_device) fx_check(fx_buf_write_box(lbm->ctx, device, host.data(), (uint)sizeof(T), N, d, Nx, Ny, x0, x1, y0, y1, z0, z1, dimension, blocking ? 1 : 0), "writing " + name);
 	else {
 		refresh();
@@ -249,7 +311,15 @@ template<typename T> void Memory_Container<T>::write_to_device_2d(const ulong x0
 template<typename T> void Memory_Container<T>::read_from_device_3d(const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong z0, const ulong z1, const ulong Nx, const ulong Ny, const ulong Nz, const int dimension, const bool blocking) { (void)Nz; box(false, x0, x1, y0, y1, z0, z1, Nx, Ny, dimension, blocking); }
 template<typename T> void Memory_Container<T>::write_to_device_3d(const ulong x0, const ulong x1, const ulong y0, const ulong y1, const ulong z0, const ulong z1, const ulong Nx, const ulong Ny, const ulong Nz, const int dimension, const bool blocking) { (void)Nz; box(true, x0, x1, y0, y1, z0, z1, Nx, Ny, dimension, blocking); }
 template<typename T> void Memory_Container<T>::write_to_device() {
-	fx_check(fx_buf_write(lbm->ctx, device, 0ull, capacity(), host.data(), 1), "writing " + name);
+	if(lbm->domain.size() == 1u) {
+		fx_check(fx_buf_write(lbm->ctx, device, 0ull, capacity(), host.data(), 1), "writing " + name);
+		return;
+	}
+	vector<T> local;
+	for(size_t k = 0u; k < lbm->domain.size(); k++) {
+		gather(lbm->domain[k], local);
+		fx_check(fx_buf_write(lbm->domain[k].ctx, devices[k], 0ull, local.size() * sizeof(T), local.data(), 1), "writing " + name);
+	}
 }
 // `path` is a prefix, by default the executable's export/ folder: the file is
 // <path><name>-<t as 9 digits>.vtk (default_filename, src/lbm.hpp:398-404)

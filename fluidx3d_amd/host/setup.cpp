@@ -52,6 +52,7 @@ void main_setup() {
 #ifdef FX_VTK
 	lbm.u.write_host_to_vtk("export/", false); // export/u-<t>.vtk
 #endif
+	lbm.write_status("export/"); // export/status-<t>.txt
 }
 
 #elif defined(SCENE_SPHERE)
@@ -141,6 +142,49 @@ void main_setup() {
 	const float3 F = lbm.object_force(TYPE_S|TYPE_X); // no step since the last check: the same force
 	char line[256];
 	std::snprintf(line, sizeof(line), "t=%llu F=(%.9e, %.9e, %.9e) F_Stokes=%.9e", (unsigned long long)lbm.get_t(), (double)F.x, (double)F.y, (double)F.z, (double)units.F_Stokes(rho, u, nu, R));
+	println(line);
+}
+
+#elif defined(SCENE_DOMAINS)
+#ifndef DOMAINS_X
+#define DOMAINS_X 1u
+#endif
+#ifndef DOMAINS_Y
+#define DOMAINS_Y 1u
+#endif
+#ifndef DOMAINS_Z
+#define DOMAINS_Z 1u
+#endif
+// A 24x18x12 lattice split into DOMAINS_X x DOMAINS_Y x DOMAINS_Z domains on one device
+// (the reference's LBM(Nx,Ny,Nz,Dx,Dy,Dz,...) constructor, src/lbm.cpp:721-773): seeded
+// random rho/u/F, scattered TYPE_S cells, TYPE_E planes at x=0 and x=Nx-1, a volume force.
+// Writes the initial fields and rho/u after 9 steps as VTK to export/, prints the FP64
+// sums.  Needs EQUILIBRIUM_BOUNDARIES, VOLUME_FORCE and FORCE_FIELD.
+void main_setup() {
+	LBM lbm(24u, 18u, 12u, DOMAINS_X, DOMAINS_Y, DOMAINS_Z, 0.05f, 1E-5f, -2E-5f, 5E-6f);
+	const uint Nx = lbm.get_Nx();
+	uint seed = 7u;
+	for(ulong n = 0ull; n < lbm.get_N(); n++) { // sequential: the LCG draws are ordered
+		uint x = 0u, y = 0u, z = 0u; lbm.coordinates(n, x, y, z);
+		lbm.rho[n] = 1.0f + random_symmetric(seed, 0.01f);
+		const float ux = random_symmetric(seed, 0.05f), uy = random_symmetric(seed, 0.05f), uz = random_symmetric(seed, 0.05f);
+		lbm.F.x[n] = random_symmetric(seed, 1E-4f); lbm.F.y[n] = random_symmetric(seed, 1E-4f); lbm.F.z[n] = random_symmetric(seed, 1E-4f);
+		if(random(seed, 1.0f) < 0.08f) { lbm.flags[n] = TYPE_S; continue; }
+		lbm.u.x[n] = ux; lbm.u.y[n] = uy; lbm.u.z[n] = uz;
+		if(x == 0u || x == Nx - 1u) lbm.flags[n] = TYPE_E;
+	}
+	lbm.rho.write_host_to_vtk("export/", false);
+	lbm.u.write_host_to_vtk("export/", false);
+	lbm.flags.write_host_to_vtk("export/", false);
+	lbm.F.write_host_to_vtk("export/", false);
+	lbm.run(9u);
+	lbm.rho.write_device_to_vtk("export/", false);
+	lbm.u.write_device_to_vtk("export/", false);
+	const double mass = lbm.total_mass();
+	const float3 F = lbm.object_force(TYPE_S), M = lbm.object_torque(float3(1.5f, -2.0f, 0.5f), TYPE_S), c = lbm.object_center_of_mass(TYPE_S);
+	char line[512];
+	std::snprintf(line, sizeof(line), "domains %ux%ux%u t=%llu mass=%.17g force=(%.9e, %.9e, %.9e) torque=(%.9e, %.9e, %.9e) com=(%.9e, %.9e, %.9e)", lbm.get_Dx(), lbm.get_Dy(), lbm.get_Dz(),
+	              (unsigned long long)lbm.get_t(), mass, (double)F.x, (double)F.y, (double)F.z, (double)M.x, (double)M.y, (double)M.z, (double)c.x, (double)c.y, (double)c.z);
 	println(line);
 }
 

@@ -56,6 +56,16 @@ def test_cpp_taylor_green_matches_python_path(tmp_path):
     assert m and abs(float(m.group(1))) < 1e-6
     dims, u_cpp = read_vtk(os.path.join(tmp_path, "export", "u-000001000.vtk"))
     assert dims == [64, 64, 64]
+    # LBM::write_status (src/lbm.cpp:1047-1072): the reference's lines and number text
+    status = open(os.path.join(tmp_path, "export", "status-000001000.txt")).read().splitlines()
+    assert status[0] == "Grid Resolution = 64 x 64 x 64 = 262144"
+    assert status[1] == "Grid Domains = 1 x 1 x 1 = 1"
+    assert status[2] == "LBM Type = D3Q19 SRT"
+    assert status[4] == "Maximum Allocation Size = 19 MB"
+    assert status[5] == "Time Steps = 1000 / 1000"
+    assert status[8] == "Kinematic Viscosity = 1.66666672E-1"  # to_string(1.0f/6.0f), src/utilities.hpp:2745-2754
+    assert status[9] == "Relaxation Time = 1.00000000"
+    assert re.fullmatch(r"Average MLUPs/s = \d+", status[7]) and int(status[7].split()[-1]) > 1000
     N = 64 ** 3
     lbm = fx.LBM(64, 64, 64, 1.0 / 6.0)
     rho, u, flags = scenes.taylor_green(64, 64, 64)
