@@ -208,7 +208,7 @@ def adjoint_timing(fx, dims, steps, window, interval):
     from fluidx3d_amd import _abi
     from fluidx3d_amd.adjoint import Adjoint
     from tests import scenes
-    rho, u, flags, alpha = scenes.channel_c4(*dims)
+    rho, u, flags, alpha = scenes.porous_channel(*dims)
     lbm = fx.LBM(*dims, 1.0 / 6.0, 1e-6, 0.0, 0.0, extensions=("VOLUME_FORCE", "POROSITY", "PRECISE_DENSITY"))
     lbm.rho.host[:] = rho; lbm.u.host[:] = u; lbm.flags.host[:] = flags; lbm.porosity.host[:] = alpha
     lbm.initialize()

@@ -62,10 +62,15 @@ void main_setup() {
 #ifndef SPHERE_SCALE
 #define SPHERE_SCALE 1u
 #endif
+#ifndef SPHERE_REPORT
+#define SPHERE_REPORT 1000u
+#endif
 // Flow past a sphere: 512x256x256 (times SPHERE_SCALE), radius 32*scale at
 // (128,128,128)*scale, TYPE_E inflow/outflow u=(0.075,0,0) at x=0 and x=Nx-1, periodic in y
-// and z, Re=100 on the diameter, seeded 1e-4 perturbation of u; drag from the FP64
-// object force sum.  Needs EQUILIBRIUM_BOUNDARIES and FORCE_FIELD.
+// and z, Re=100 on the diameter, seeded 1e-4 perturbation of u.  Needs
+// EQUILIBRIUM_BOUNDARIES.  With FORCE_FIELD the drag (FP64 object force sum) is printed
+// every SPHERE_REPORT steps (and at the end); after the run, the FP64 total mass against its
+// value after initialization and the number of non-finite velocity components.
 void main_setup() {
 	const uint s = SPHERE_SCALE;
 	const float R = 32.0f * (float)s, u_in = 0.075f, Re = 100.0f;
@@ -81,15 +86,24 @@ void main_setup() {
 		lbm.u.x[n] = u_in + px; lbm.u.y[n] = py; lbm.u.z[n] = pz;
 		if(x == 0u || x == Nx - 1u) { lbm.flags[n] = TYPE_E; lbm.u.x[n] = u_in; lbm.u.y[n] = 0.0f; lbm.u.z[n] = 0.0f; }
 	}
-	Clock clock;
-	lbm.run(SPHERE_STEPS);
-#ifdef FORCE_FIELD
-	const float3 F = lbm.object_force(TYPE_S | TYPE_X);
-	const double Cd = (double)F.x / (0.5 * (double)sq(u_in) * pi * (double)sq(R));
+	lbm.run(0u); // initialize
+	const double mass0 = lbm.total_mass();
 	char line[256];
-	std::snprintf(line, sizeof(line), "t=%llu drag F=(%.9e, %.9e, %.9e) Cd=%.6f MLUPs/s=%.0f", (unsigned long long)lbm.get_t(), (double)F.x, (double)F.y, (double)F.z, Cd, (double)lbm.get_N() * (double)lbm.get_t() * 1E-6 / info.runtime_lbm);
-	println(line);
+	while(lbm.get_t() < (ulong)SPHERE_STEPS) {
+		lbm.run(std::min<ulong>((ulong)SPHERE_REPORT, (ulong)SPHERE_STEPS - lbm.get_t()));
+#ifdef FORCE_FIELD
+		const float3 F = lbm.object_force(TYPE_S | TYPE_X);
+		const double Cd = (double)F.x / (0.5 * (double)sq(u_in) * pi * (double)sq(R));
+		std::snprintf(line, sizeof(line), "t=%llu drag F=(%.9e, %.9e, %.9e) Cd=%.6f MLUPs/s=%.0f", (unsigned long long)lbm.get_t(), (double)F.x, (double)F.y, (double)F.z, Cd, (double)lbm.get_N() * (double)lbm.get_t() * 1E-6 / info.runtime_lbm);
+		println(line);
 #endif
+	}
+	const double mass1 = lbm.total_mass();
+	lbm.u.read_from_device();
+	ulong nonfinite = 0ull;
+	for(ulong i = 0ull; i < 3ull * lbm.get_N(); i++) nonfinite += !std::isfinite(lbm.u.data()[i]);
+	std::snprintf(line, sizeof(line), "mass %.12e -> %.12e (relative change %.3e), non-finite u components %llu", mass0, mass1, (mass1 - mass0) / mass0, (unsigned long long)nonfinite);
+	println(line);
 }
 
 #elif defined(SCENE_STOKES)

@@ -1,5 +1,5 @@
-"""Writes tests/golden/adjoint_c4_fd.json: central finite differences of the adjoint
-objective of the 128x64x64 porous channel (tests/scenes.py C4) with respect to the
+"""Writes tests/golden/adjoint_channel_fd.json: central finite differences of the adjoint
+objective of the 128x64x64 porous channel (tests/scenes.py CHANNEL_128) with respect to the
 porosity of 32 seeded cells, from the FP64 oracle (double arithmetic and storage).
 
     python tests/golden/make_adjoint_fd.py [--threads 8] [--scale 1]
@@ -37,14 +37,14 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=os.cpu_count())
     ap.add_argument("--scale", type=int, default=1)
-    ap.add_argument("--out", default=os.path.join(ROOT, "tests", "golden", "adjoint_c4_fd.json"))
+    ap.add_argument("--out", default=os.path.join(ROOT, "tests", "golden", "adjoint_channel_fd.json"))
     a = ap.parse_args()
-    cfg = dict(scenes.C4)
+    cfg = dict(scenes.CHANNEL_128)
     if a.scale > 1:
         for k in ("Nx", "Ny", "Nz", "steps", "window_begin", "checkpoint_interval"):
             cfg[k] //= a.scale
         cfg["plane_x"] = cfg["Nx"] - 1
-    rho, u, flags, alpha = scenes.channel_c4(cfg["Nx"], cfg["Ny"], cfg["Nz"])
+    rho, u, flags, alpha = scenes.porous_channel(cfg["Nx"], cfg["Ny"], cfg["Nz"])
     cells = scenes.fd_cells(flags, alpha, cfg["fd_cells"], cfg["fd_seed"])
     eps = cfg["fd_eps"]
     t0 = time.time()

@@ -76,7 +76,7 @@ def porosity_noise(Nx, Ny, Nz, x0, x1, sigma=4.0, seed=42):
     return out.ravel()
 
 
-def channel_c4(Nx=128, Ny=64, Nz=64, sigma=4.0, seed=42):
+def porous_channel(Nx=128, Ny=64, Nz=64, sigma=4.0, seed=42):
     """Adjoint channel: TYPE_S walls at y=0 and y=Ny-1, periodic in x and z, fluid at rest,
     Brinkman porosity in [0,1] from porosity_noise on the middle third of x (zero on the
     walls).  Returns (rho, u, flags, porosity) as float32/uint8 arrays."""
@@ -91,7 +91,7 @@ def channel_c4(Nx=128, Ny=64, Nz=64, sigma=4.0, seed=42):
 
 # adjoint configuration of the 128x64x64 channel: nu=1/6, body force 1e-6 in x, 2000 steps,
 # objective = outlet-plane (x = Nx-1) x-momentum averaged over the last 500 steps
-C4 = {"Nx": 128, "Ny": 64, "Nz": 64, "nu": 1.0 / 6.0, "fx": 1e-6, "steps": 2000, "window_begin": 1500,
+CHANNEL_128 = {"Nx": 128, "Ny": 64, "Nz": 64, "nu": 1.0 / 6.0, "fx": 1e-6, "steps": 2000, "window_begin": 1500,
       "plane_x": 127, "checkpoint_interval": 100, "fd_cells": 32, "fd_seed": 7, "fd_eps": 1e-3}
 
 

@@ -102,7 +102,7 @@ def test_adjoint_step_matches_fp64_oracle(oracle, q, collision, variant, grid):
 
 def small_channel(fx_value=1e-5, dims=(32, 16, 12)):
     Nx, Ny, Nz = dims
-    rho, u, flags, alpha = scenes.channel_c4(Nx, Ny, Nz)
+    rho, u, flags, alpha = scenes.porous_channel(Nx, Ny, Nz)
     return (Nx, Ny, Nz), rho, u, flags, alpha
 
 
@@ -149,16 +149,16 @@ def test_checkpoint_interval_does_not_change_the_gradient():
     assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
 
 
-def test_c4_gradient_matches_finite_differences():
+def test_channel_128_gradient_matches_finite_differences():
     """128x64x64 channel, 2000 steps, outlet momentum averaged over the last 500 steps,
     checkpoints every 100: GPU adjoint vs central differences of the FP64 oracle on 32
-    seeded cells (tests/golden/adjoint_c4_fd.json, made by tests/golden/make_adjoint_fd.py)."""
-    path = os.path.join(ROOT, "tests", "golden", "adjoint_c4_fd.json")
+    seeded cells (tests/golden/adjoint_channel_fd.json, made by tests/golden/make_adjoint_fd.py)."""
+    path = os.path.join(ROOT, "tests", "golden", "adjoint_channel_fd.json")
     if not os.path.exists(path):
-        pytest.skip("tests/golden/adjoint_c4_fd.json not generated")
+        pytest.skip("tests/golden/adjoint_channel_fd.json not generated")
     gold = json.load(open(path))
     c = gold["config"]
-    rho, u, flags, alpha = scenes.channel_c4(c["Nx"], c["Ny"], c["Nz"])
+    rho, u, flags, alpha = scenes.porous_channel(c["Nx"], c["Ny"], c["Nz"])
     J, g = gpu_gradient((c["Nx"], c["Ny"], c["Nz"]), rho, u, flags, alpha, c["steps"], c["window_begin"], c["plane_x"],
                         c["checkpoint_interval"], c["fx"])
     fd = np.array(gold["fd"])

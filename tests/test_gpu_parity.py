@@ -169,7 +169,7 @@ def test_velocity_clamp_matches_oracle(oracle, q, collision, storage, ext, grid)
 
 @pytest.mark.parametrize("storage", ["FP32", "FP16S", "FP16C"])
 def test_tgv_64_1000_steps(oracle, storage):
-    """Config C1 shape: D3Q19 SRT 64^3 Taylor-Green, nu=1/6, u0=0.05, 1000 steps.
+    """D3Q19 SRT 64^3 Taylor-Green, nu=1/6, u0=0.05, 1000 steps.
     GPU == FP32 oracle (same storage) exactly; the error against the FP64 oracle is stated
     and bounded relative to u0; the kinetic energy decays at the viscous rate 2*nu*|k|^2."""
     Nx = Ny = Nz = 64
@@ -204,7 +204,7 @@ def test_tgv_64_1000_steps(oracle, storage):
 
 
 def test_full_size_256_two_steps(oracle):
-    """Config C2 size: 256^3 D3Q19 SRT FP32 TGV, 2 steps, every cell compared with the oracle."""
+    """256^3 D3Q19 SRT FP32 TGV, 2 steps, every cell compared with the oracle."""
     Nx = Ny = Nz = 256
     rho, u, flags = scenes.taylor_green(Nx, Ny, Nz)
     lbm, ref = setup_pair(oracle, Nx, Ny, Nz, 1.0 / 6.0, 19, "SRT", "FP32", (), rho, u, flags)

@@ -92,16 +92,32 @@ def test_cpp_benchmark_scene(tmp_path, name):
     assert mlups > floor
 
 
+DRAG = re.compile(r"t=(\d+) drag F=\(([-0-9.e+naif]+), ([-0-9.e+naif]+), ([-0-9.e+naif]+)\) Cd=([-0-9.e+naif]+)")
+MASS = re.compile(r"mass ([-0-9.e+naif]+) -> ([-0-9.e+naif]+) \(relative change ([-0-9.e+naif]+)\), non-finite u components (\d+)")
+
+
+def sphere_run(tmp_path, name, timeout=600):
+    """Runs a sphere scene binary; returns its drag reports [(t, Fx, Fy, Fz, Cd)], the mass
+    line (m0, m1, relative change, non-finite u components) and stdout."""
+    r = subprocess.run([binary(name)], cwd=tmp_path, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    drags = [(int(m.group(1)),) + tuple(float(v) for v in m.groups()[1:]) for m in DRAG.finditer(r.stdout)]
+    m = MASS.search(r.stdout)
+    assert m, r.stdout[-2000:]
+    mass = tuple(float(v) for v in m.groups()[:3]) + (int(m.group(4)),)
+    return drags, mass, r.stdout
+
+
 @pytest.mark.gpu
 def test_cpp_sphere_scene(tmp_path):
-    """D3Q27 TRT FP16C sphere with TYPE_E inflow/outflow: 1000 steps, finite positive drag."""
-    r = subprocess.run([binary("sphere")], cwd=tmp_path, capture_output=True, text=True, timeout=300)
-    assert r.returncode == 0, r.stderr
-    m = re.search(r"drag F=\(([-0-9.e+]+), ([-0-9.e+]+), ([-0-9.e+]+)\) Cd=([-0-9.e+]+)", r.stdout)
-    assert m, r.stdout
-    Fx, Fy, Fz, Cd = (float(v) for v in m.groups())
-    print("\n" + r.stdout.strip().splitlines()[-1])
+    """D3Q27 TRT FP16C sphere with TYPE_E inflow/outflow: 1000 steps, finite positive drag,
+    finite velocities, FP64 mass within 1e-3 of its initial value."""
+    drags, mass, out = sphere_run(tmp_path, "sphere")
+    t, Fx, Fy, Fz, Cd = drags[-1]
+    print("\n" + "\n".join(out.strip().splitlines()[-2:]))
+    assert t == 1000
     assert np.isfinite([Fx, Fy, Fz, Cd]).all() and Fx > 0 and abs(Fy) < 0.1 * Fx and abs(Fz) < 0.1 * Fx
+    assert mass[3] == 0 and abs(mass[2]) < 1e-3
 
 
 def lcg_channel_porosity(Nx, Ny, Nz, seed=42):

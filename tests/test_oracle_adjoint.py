@@ -10,7 +10,7 @@ from tests import scenes
 
 def channel(oracle, q, trt, ext, alpha_override=None):
     Nx, Ny, Nz = 14, 8, 6
-    rho, u, flags, alpha = scenes.channel_c4(Nx, Ny, Nz)
+    rho, u, flags, alpha = scenes.porous_channel(Nx, Ny, Nz)
     rng = np.random.default_rng(3)
     alpha = np.where(alpha > 0, alpha, 0.0) + 0.2 * rng.uniform(0, 1, alpha.size) * ((flags & 1) == 0)
     if ext & oracle.EXT_EQUILIBRIUM_BOUNDARIES:  # inflow plane with a preset velocity

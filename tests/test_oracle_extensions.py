@@ -50,11 +50,11 @@ def test_smagorinsky_channel_follows_momentum_balance(oracle):
     L.step(100000)
     L.update_fields()
     ux = L.u[:N].reshape(4, Ny, 4).mean(axis=(0, 2))
-    C2 = 0.76421222 / (18.0 * np.sqrt(2.0))
+    CS2 = 0.76421222 / (18.0 * np.sqrt(2.0))
     yb, yt = 0.5, Ny - 1.5
     yy = np.linspace(yb, yt, 20001)
     s = -f * (yy - 0.5 * (yb + yt))
-    gamma = np.sign(s) * (-nu0 + np.sqrt(nu0 ** 2 + 4.0 * C2 * np.abs(s))) / (2.0 * C2)
+    gamma = np.sign(s) * (-nu0 + np.sqrt(nu0 ** 2 + 4.0 * CS2 * np.abs(s))) / (2.0 * CS2)
     u = np.concatenate([[0.0], np.cumsum(0.5 * (gamma[1:] + gamma[:-1]) * np.diff(yy))])
     exact = np.interp(np.arange(Ny), yy, u)
     laminar_peak = f * (0.5 * (yt - yb)) ** 2 / (2.0 * nu0)

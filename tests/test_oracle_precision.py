@@ -10,7 +10,7 @@ CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0])
 
 
 def objective(oracle, storage, ext, dims=(48, 24, 16), steps=500, window=400):
-    rho, u, flags, alpha = scenes.channel_c4(*dims)
+    rho, u, flags, alpha = scenes.porous_channel(*dims)
     L = oracle.OracleLBM(*dims, nu=1.0 / 6.0, storage=storage, ext=ext, fx=float(np.float32(1e-6)))
     L.rho[:] = rho; L.u[:] = u; L.flags[:] = flags; L.alpha[:] = alpha
     L.initialize()

@@ -30,7 +30,7 @@ def make_lbm(dims, rho, u, flags, fxv):
 def main():
     dims = (32, 16, 12)
     steps, window, plane, fxv = 300, 200, dims[0] - 1, 1e-5
-    rho, u, flags, alpha = scenes.channel_c4(*dims)
+    rho, u, flags, alpha = scenes.porous_channel(*dims)
     lbm = make_lbm(dims, rho, u, flags, fxv)
     adj = Adjoint(lbm, steps, window, plane, 30)
     p = torch.tensor(alpha, device="cuda", requires_grad=True)
