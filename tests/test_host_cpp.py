@@ -63,7 +63,10 @@ def test_cpp_taylor_green_matches_python_path(tmp_path):
     assert status[2] == "LBM Type = D3Q19 SRT"
     assert status[4] == "Maximum Allocation Size = 19 MB"
     assert status[5] == "Time Steps = 1000 / 1000"
-    assert status[8] == "Kinematic Viscosity = 1.66666672E-1"  # to_string(1.0f/6.0f), src/utilities.hpp:2745-2754
+    # to_string(1.0f/6.0f) of src/utilities.hpp:2745-2754: the float is scaled by 10 in float
+    # arithmetic before its digits are taken, which rounds 1.66666671... up to 1.66666675...
+    # (the reference's own helper prints the same, tests/test_host_text.py)
+    assert status[8] == "Kinematic Viscosity = 1.66666676E-1"
     assert status[9] == "Relaxation Time = 1.00000000"
     assert re.fullmatch(r"Average MLUPs/s = \d+", status[7]) and int(status[7].split()[-1]) > 1000
     N = 64 ** 3

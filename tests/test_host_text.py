@@ -76,6 +76,9 @@ def test_float_and_double_text_like_reference(host, ref):
                               np.array([0.0, 1.0, 0.1, 1e256, 1e-300, 123456.789, np.inf, np.nan])])
     for x in doubles:
         assert text(host.host_to_string_double, x) == text(ref.ref_to_string_double, x), x
+    # the viscosity line of write_status for nu = 1/6 (the C++ scene test checks this text)
+    sixth = np.float32(1.0) / np.float32(6.0)
+    assert text(ref.ref_to_string_float, sixth) == text(host.host_to_string_float, sixth) == "1.66666676E-1"
 
 
 def test_fixed_decimals_like_reference(host, ref):
