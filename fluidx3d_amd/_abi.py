@@ -41,7 +41,7 @@ EXPORTS = [
     "fx_lbm_mass_sum", "fx_codec_roundtrip", "fx_bench_copy", "fx_adjoint_memory", "fx_adjoint_create",
     "fx_adjoint_destroy", "fx_lbm_forward", "fx_lbm_backward", "fx_lbm_stream_collide_record", "fx_lbm_adjoint_step",
     "fx_lbm_objective_plane", "fx_transfer_bytes", "fx_lbm_transfer_extract", "fx_lbm_transfer_insert",
-    "fx_lbm_update_moving_boundaries", "fx_buf_read_box", "fx_buf_write_box",
+    "fx_lbm_update_moving_boundaries", "fx_buf_read_box", "fx_buf_write_box", "fx_lbm_mass_sum_noalloc",
 ]
 
 
@@ -114,6 +114,7 @@ def load():
     L.fx_lbm_object_torque.argtypes = [vp, vp, vp, ctypes.c_uint8, f32, f32, f32, dp]
     L.fx_lbm_object_center_of_mass.argtypes = [vp, vp, ctypes.c_uint8, dp]
     L.fx_lbm_mass_sum.argtypes = [vp, vp, vp, u64, dp]
+    L.fx_lbm_mass_sum_noalloc.argtypes = [vp, vp, vp, u64, dp]
     L.fx_codec_roundtrip.argtypes = [vp, u32, vp, vp, vp, u64]
     acp = ctypes.POINTER(AdjointConfig)
     fp = ctypes.POINTER(f32)

@@ -46,6 +46,7 @@ struct fx_ctx {
 	double* partial = nullptr; // reduction partials (device)
 	uint64_t partial_cap = 0;  // in doubles
 	double* result = nullptr;  // 4 doubles (device)
+	double* scratch = nullptr; // this device's copy of the static scratch array (fx_lbm_mass_sum_noalloc)
 	std::set<void*> buffers;   // allocations owned by this context
 };
 
@@ -66,6 +67,33 @@ namespace {
 
 int set_device(const fx_ctx* ctx) {
 	FX_HIP(hipSetDevice(ctx->device));
+	return FX_OK;
+}
+
+// Scratch memory of fx_lbm_mass_sum_noalloc, for callers that cannot allocate: the FP64
+// per-block partials (at most NOALLOC_MASS_PARTIALS) followed by the one-double result.
+// Every device the library runs on holds its own copy (8 MiB) until the process ends, and
+// every call on a device shares it.  Host code never passes the array's name to a kernel or
+// copy (on the host it is a stand-in, not a device address): it uses the device address
+// that resolve_scratch obtains once per device with that device current.
+constexpr uint64_t NOALLOC_MASS_PARTIALS = 1ull << 20;
+static __device__ __align__(256) double scratch[NOALLOC_MASS_PARTIALS + 1];
+
+std::mutex g_scratch_mutex;
+double* g_scratch_address[256] = {}; // per device ordinal
+
+// Device address of `scratch` on `device` (which must be current), resolved on the first
+// call for that device.
+int resolve_scratch(int device, double** out) {
+	if(device < 0 || device >= 256) return fail(FX_ERR_ARG, "device ordinal out of range");
+	std::lock_guard<std::mutex> lock(g_scratch_mutex);
+	if(!g_scratch_address[device]) {
+		void* p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if(e != hipSuccess) return fail_hip(e, "hipGetSymbolAddress(scratch)");
+		g_scratch_address[device] = (double*)p;
+	}
+	*out = g_scratch_address[device];
 	return FX_OK;
 }
 
@@ -451,6 +479,11 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 		fx_ctx_destroy(ctx);
 		return fail_hip(e, "fx_ctx_create");
 	}
+	if(resolve_scratch(device, &ctx->scratch) != FX_OK) { // device is current here
+		const std::string msg = g_last_error;
+		fx_ctx_destroy(ctx);
+		return fail(FX_ERR_HIP, msg);
+	}
 	*out = ctx;
 	return FX_OK;
 }
@@ -666,6 +699,20 @@ int fx_lbm_mass_sum(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t 
 	const int rc = ensure_partial(ctx, blocks);
 	if(rc != FX_OK) return rc;
 	return mass_sum_with(ctx, fi, flags, t, ctx->partial, ctx->result, out);
+}
+
+// The same sum over the static scratch array: no allocation, no clearing needed (every
+// partial the sum reads and the result are written by this call's kernels).  Calls that
+// overlap on one device (contexts on different streams, or threads) race on the array.
+int fx_lbm_mass_sum_noalloc(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t t, double* out) {
+	if(!ctx || !fi || !flags || !out) return fail(FX_ERR_ARG, "null argument");
+	if(!ctx->scratch) return fail(FX_ERR_STATE, "the scratch array's device address is not resolved");
+	const uint64_t blocks = (uint64_t)ctx->shape.grid.x * ctx->shape.grid.y * ctx->shape.grid.z;
+	if(blocks > NOALLOC_MASS_PARTIALS)
+		return fail(FX_ERR_CONFIG, "fx_lbm_mass_sum_noalloc: the lattice needs " + std::to_string(blocks) + " partial sums, the static scratch holds " +
+		                           std::to_string(NOALLOC_MASS_PARTIALS) + "; use fx_lbm_mass_sum");
+	if(set_device(ctx) != FX_OK) return FX_ERR_HIP;
+	return mass_sum_with(ctx, fi, flags, t, ctx->scratch, ctx->scratch + NOALLOC_MASS_PARTIALS, out);
 }
 
 static uint64_t ddf_bytes(const fx_ctx* ctx) {

@@ -214,6 +214,16 @@ int fx_lbm_object_center_of_mass(fx_ctx* ctx, const uint8_t* flags, uint8_t mark
  * counterpart; the conservation check. */
 int fx_lbm_mass_sum(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t t, double* out);
 
+/* fx_lbm_mass_sum without any allocation: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The partial sums and the result live in a static
+ * device array of the library (8 MiB per device, held until the process ends), so the
+ * lattice's grid may have at most 2^20 blocks (FX_ERR_CONFIG beyond; about 268 M cells
+ * with 256-cell rows).  Same result as fx_lbm_mass_sum, bit for bit.  All calls on one
+ * device share the array: calls that overlap on one device (other contexts' streams,
+ * other threads) race and must be serialised by the caller. */
+int fx_lbm_mass_sum_noalloc(fx_ctx* ctx, const void* fi, const uint8_t* flags, uint64_t t, double* out);
+
 /* ---- differentiation (no reference counterpart; the reference has no adjoint) ----
  *
  * Objective J = mean over steps t in [window_begin, steps) of the x-momentum
