@@ -795,6 +795,110 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false, bool LIN 
 __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
 	stream_collide_cell<Q, ST, TRT, EXT, false, XCD, LIN>(a, nullptr);
 }
+// stream_collide with two adjacent cells per lane, for the 16-bit formats on rows that
+// start on 128-byte lines (Nx a multiple of 64).  Lane (chunk, y, z) takes cells
+// xa = 2*(chunk*blockDim.x + lane) and xa+1.  Each unshifted slot and each slot shifted
+// only in y/z is one 32-bit word holding both cells' codes (low half: cell xa), so those
+// accesses are 4 bytes per lane instead of 2; the slots shifted in x take one 16-bit
+// access per cell (xa's neighbour and xa+1's are not one aligned word, and the one at the
+// row's end wraps).  The per-lane words are decoded, collided and re-encoded one cell at a
+// time, so only one cell's floats are live.  A cell that does not collide (solid or gas)
+// writes each of its codes back to where it was loaded from: Esoteric-Pull stores a
+// population to the other location of its pair, so the codes of each pair are swapped
+// first, and the cell's memory is left as it was.  Same results as k_stream_collide, bit
+// for bit.
+//
+// The 16-bit kernels keep the texture address unit 81 % (FP16S) and 91 % (FP16C) busy at
+// about 10 cycles per wave access whether it moves 2 or 4 bytes per lane, so fewer, wider
+// accesses raise the rate.  Measured at 256^3 against k_stream_collide, same process,
+// D3Q19 SRT: FP16S none +2.5 %, EQUILIBRIUM_BOUNDARIES +5.0 %, + FORCE_FIELD +5.9 %;
+// FP16C none +1.2 %, EQUILIBRIUM_BOUNDARIES +5.4 %; 1536x768x768 FP16S
+// EQUILIBRIUM_BOUNDARIES +3.9 %; with 1/64 of the cells solid and scattered, +13 to
+// +19 % (a solid lane no longer leaves a hole in every access of its wave).  The TRT
+// collisions hold 73 VGPRs and gained nothing (FP16C -4 %), D3Q27 needs 133-150 VGPRs
+// (-7 to -25 %); those keep one cell per lane (kernels_for, stream_collide_pairs).
+typedef __attribute__((address_space(1))) uint16_t gcode;
+typedef __attribute__((address_space(1))) uint32_t gword;
+// Wave-uniform base of a slot row, in scalar registers and in the global address space
+// (readfirstlane returns an integer; without the address space the accesses would be
+// generic flat ones).
+__device__ __forceinline__ gcode* global_row(const uint16_t* p) {
+	const uint64_t v = (uint64_t)p;
+	const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+	return (gcode*)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ gcode* code_at(gcode* p, uint32_t x) { return (gcode*)((__attribute__((address_space(1))) char*)p + x * 2u); }
+__device__ __forceinline__ gword* word_at(gcode* p, uint32_t x) { return (gword*)code_at(p, x); } // x even
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT>
+__global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
+	a.ext &= EXT;
+	typedef Storage<ST> S;
+	uint32_t chunk, y, z;
+	if(!block_coords<true>(a, chunk, y, z)) return;
+	const uint32_t xa = 2u * (chunk * blockDim.x + threadIdx.x);
+	if(xa >= a.Nx) return;
+	const Site s = make_site(xa, y, z, a.Nx, a.Ny, a.Nz); // X = {xa-1, xa, xa+1}: the first cell
+	const uint32_t XB[3] = {xa, xa + 1u, xa + 2u == a.Nx ? 0u : xa + 2u}; // the second cell
+	const uint64_t n = s.n();
+	const uint32_t fl = *(const uint16_t*)(a.flags + n); // n is even: flags[n] low, flags[n+1] high
+	const uint32_t flA = fl & 0xFFu, flB = fl >> 8;
+	const bool liveA = !((flA & TYPE_BO) == TYPE_S || (flA & TYPE_SU) == TYPE_G);
+	const bool liveB = !((flB & TYPE_BO) == TYPE_S || (flB & TYPE_SU) == TYPE_G);
+	if(!liveA && !liveB) return;
+	uint16_t* fi = (uint16_t*)a.fi;
+	const uint64_t N = a.N, n0 = s.R[4];
+	uint32_t P[Q]; // population i of both cells, as loaded (load_raw's order)
+	__builtin_amdgcn_s_setprio(1);
+	P[0] = __builtin_nontemporal_load(word_at(global_row(fi + n0), xa));
+#pragma unroll
+	for(int i = 1; i < Q; i += 2) {
+		const uint64_t sa = a.odd ? (uint64_t)i : (uint64_t)(i + 1);
+		const uint64_t sb = a.odd ? (uint64_t)(i + 1) : (uint64_t)i;
+		P[i] = __builtin_nontemporal_load(word_at(global_row(fi + sa * N + n0), xa));
+		gcode* pb = global_row(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]));
+		if(VS<Q>::cx[i] == 0) P[i + 1] = __builtin_nontemporal_load(word_at(pb, xa));
+		else P[i + 1] = (uint32_t)*code_at(pb, s.X[VS<Q>::cx[i] + 1]) | ((uint32_t)*code_at(pb, XB[VS<Q>::cx[i] + 1]) << 16);
+	}
+	__builtin_amdgcn_sched_barrier(0);
+	__builtin_amdgcn_s_setprio(0);
+	float f[Q];
+	if(liveA) {
+#pragma unroll
+		for(int i = 0; i < Q; i++) f[i] = S::decode((uint16_t)P[i]);
+		collide_cell<Q, TRT, EXT, false>(a, n, flA & TYPE_BO, f, nullptr);
+#pragma unroll
+		for(int i = 0; i < Q; i++) P[i] = (P[i] & 0xFFFF0000u) | (uint32_t)S::encode(f[i]);
+	} else {
+#pragma unroll
+		for(int i = 1; i < Q; i += 2) { const uint32_t t = (P[i] ^ P[i + 1]) & 0x0000FFFFu; P[i] ^= t; P[i + 1] ^= t; }
+	}
+	__builtin_amdgcn_sched_barrier(0); // one cell's floats at a time
+	if(liveB) {
+#pragma unroll
+		for(int i = 0; i < Q; i++) f[i] = S::decode((uint16_t)(P[i] >> 16));
+		collide_cell<Q, TRT, EXT, false>(a, n + 1u, flB & TYPE_BO, f, nullptr);
+#pragma unroll
+		for(int i = 0; i < Q; i++) P[i] = (P[i] & 0x0000FFFFu) | ((uint32_t)S::encode(f[i]) << 16);
+	} else {
+#pragma unroll
+		for(int i = 1; i < Q; i += 2) { const uint32_t t = (P[i] ^ P[i + 1]) & 0xFFFF0000u; P[i] ^= t; P[i + 1] ^= t; }
+	}
+	// store_f's order: population i to the shifted location, i+1 to the cell's own slot
+	__builtin_nontemporal_store(P[0], word_at(global_row(fi + n0), xa));
+#pragma unroll
+	for(int i = 1; i < Q; i += 2) {
+		const uint64_t sa = a.odd ? (uint64_t)i : (uint64_t)(i + 1);
+		const uint64_t sb = a.odd ? (uint64_t)(i + 1) : (uint64_t)i;
+		gcode* pb = global_row(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]));
+		if(VS<Q>::cx[i] == 0) __builtin_nontemporal_store(P[i], word_at(pb, xa));
+		else {
+			*code_at(pb, s.X[VS<Q>::cx[i] + 1]) = (uint16_t)P[i];
+			*code_at(pb, XB[VS<Q>::cx[i] + 1]) = (uint16_t)(P[i] >> 16);
+		}
+		__builtin_nontemporal_store(P[i + 1], word_at(global_row(fi + sa * N + n0), xa));
+	}
+}
+
 // stream_collide that also writes the pre-collision (rho, jx, jy, jz) of every fluid cell to
 // moments[c*N+n], c = 0..3 (16 B/cell): what the adjoint step needs of the forward state.
 // EXT is the context's extension subset; MOVING_BOUNDARIES and SUBGRID are not

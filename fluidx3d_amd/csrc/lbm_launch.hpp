@@ -10,6 +10,7 @@ struct LaunchShape {
 	size_t lds = 0;       // dynamic LDS reserved per block (occupancy limit), see occupancy_lds()
 	uint32_t xchunks = 0; // 1-D XCD-mapped grid with this many x chunks per row (see block_coords)
 	bool lin = false;     // cell-linear grid (cell_coords in lbm_device.hpp), see linear_shape
+	bool pairs = false;   // two cells per lane (k_stream_collide_pair), see pair_shape
 };
 // A launch holds at most 2^32-1 work-items along each grid dimension (grid size times
 // block size); lattices of more than ~4.29 G cells (a D3Q19 FP16 lattice of up to ~5.2 G
@@ -64,6 +65,14 @@ inline LaunchShape stream_shape(uint32_t Nx, uint32_t Ny, uint32_t Nz, bool xcd_
 	return s;
 }
 
+// k_stream_collide_pair: the row grid over Nx/2 lanes per row (Nx even), XCD-mapped when a
+// row takes several blocks.
+inline LaunchShape pair_shape(uint32_t Nx, uint32_t Ny, uint32_t Nz) {
+	LaunchShape s = stream_shape(Nx / 2u, Ny, Nz, true);
+	s.pairs = true;
+	return s;
+}
+
 // Dynamic LDS per block that caps residency at `waves_per_simd` waves per SIMD (4 SIMDs
 // per CU), for a kernel that uses no LDS itself.  0 when the cap would not bind exactly.
 // A block's LDS is allocated in granules of 1/128 of the CU's LDS (1280 B of 160 KiB on
@@ -107,6 +116,7 @@ struct LbmKernels {
 	lattice_fn update_moving_boundaries;
 	bool stream_collide_linear_large; // cell-linear grid for DDF arrays above LARGE_DDF_BYTES
 	bool stream_collide_linear_xcd;   // XCD-aware segment order on the cell-linear grid
+	bool stream_collide_pairs;        // k_stream_collide_pair on rows that start on lines
 };
 
 constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY |
@@ -134,6 +144,13 @@ constexpr uint32_t tune_waves_cap(int Q, uint32_t ST, uint32_t EXT) {
 }
 constexpr bool tune_xcd_map(int Q, uint32_t ST, uint32_t EXT) {
 	return true;
+}
+// Two cells per lane (k_stream_collide_pair, where its measurements are): the D3Q19 SRT
+// 16-bit kernels without extensions, with EQUILIBRIUM_BOUNDARIES and with
+// EQUILIBRIUM_BOUNDARIES + FORCE_FIELD.
+constexpr bool tune_pairs(int Q, uint32_t ST, bool TRT, uint32_t EXT) {
+	return Q == 19 && ST != ST_FP32 && !TRT &&
+	       (EXT == 0u || EXT == EXT_EQUILIBRIUM_BOUNDARIES || EXT == (EXT_EQUILIBRIUM_BOUNDARIES | EXT_FORCE_FIELD));
 }
 // Above 4 GiB of DDFs the FP32 kernels stream faster on the cell-linear grid than on the
 // row grid, also where the rows start on lines (same residency cap).  Plain D3Q19: 384^3
@@ -164,6 +181,9 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static void stream_collide(const LbmArgs& a, const LaunchShape& s, hipStream_t st) {
 		LbmArgs b = a;
 		b.xchunks = s.xchunks;
+		if constexpr(pairs) {
+			if(s.pairs) { k_stream_collide_pair<Q, ST, TRT, EXT><<<s.grid, s.block, s.lds, st>>>(b); return; }
+		}
 		if(s.lin) k_stream_collide<Q, ST, TRT, EXT, linear_xcd, true><<<s.grid, s.block, s.lds, st>>>(b);
 		else k_stream_collide<Q, ST, TRT, EXT, xcd_map, false><<<s.grid, s.block, s.lds, st>>>(b);
 	}
@@ -193,6 +213,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static constexpr uint32_t waves_cap = tune_waves_cap(Q, ST, EXT);
 	static constexpr bool xcd_map = tune_xcd_map(Q, ST, EXT);
 	static constexpr bool linear_xcd = tune_linear_xcd(Q, ST, EXT);
+	static constexpr bool pairs = tune_pairs(Q, ST, TRT, EXT);
 	template<int AXIS> static void transfer_axis(void* fi, void* bp, void* bm, uint32_t Nx, uint32_t Ny, uint32_t Nz, bool insert, uint32_t odd, hipStream_t st) {
 		typedef typename Storage<ST>::E E;
 		const uint64_t A = AXIS == 0 ? (uint64_t)Ny * Nz : AXIS == 1 ? (uint64_t)Nx * Nz : (uint64_t)Nx * Ny;
@@ -208,7 +229,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static const LbmKernels* table() {
 		static const LbmKernels k = {initialize, stream_collide, update_fields, update_force_field, mass_partial, waves_cap,
 		                             stream_collide_record, adjoint_step, objective_partial, xcd_map, transfer_fi,
-		                             update_moving_boundaries, tune_linear_large(Q, ST, EXT), linear_xcd};
+		                             update_moving_boundaries, tune_linear_large(Q, ST, EXT), linear_xcd, pairs};
 		return &k;
 	}
 };

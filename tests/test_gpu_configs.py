@@ -17,11 +17,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN_TGV256 = os.path.join(ROOT, "tests", "golden", "tgv256_fp64.npz")
 
 # Error bounds against the FP64 oracle of the same scheme, relative to u0 for u and absolute
-# for rho, for each DDF storage mode (the FP32 arithmetic is the same in all three).  FP32
-# storage rounds each DDF to 2^-24 relative; the 16-bit formats to 2^-11 (FP16S, 10 stored
-# mantissa bits) and 2^-12 (FP16C, 11 bits) relative to the shifted DDF f - w, whose size
-# is set by the flow (here |f - w| <= w * 3 u0 ~ 1e-2).
-TGV_BOUNDS = {"FP32": (1e-5, 1e-6), "FP16S": (2e-3, 2e-5), "FP16C": (1e-3, 1e-5)}
+# for rho, for each DDF storage mode (the FP32 arithmetic is the same in all three).  With
+# FP32 arithmetic the reference's scheme forms rho = sum(f) + 1 in float, which rounds the
+# density deviation (~3 u0^2 / 16 = 5e-4 here) to 2^-24, about 1e-4 of it: the pressure
+# field, and through it u, carries errors of order 1e-5 u0 after 1000 steps (measured
+# 1.5e-5 u0 at 256^3, 2.6e-6 u0 at 64^3).  The 16-bit formats round each stored DDF to
+# 2^-11 (FP16S, 10 stored mantissa bits) and 2^-12 (FP16C, 11 bits) of the shifted DDF
+# f - w, whose size is set by the flow (here |f - w| <= w * 3 u0 ~ 1e-2).
+TGV_BOUNDS = {"FP32": (5e-5, 2e-6), "FP16S": (2e-3, 2e-5), "FP16C": (1e-3, 1e-5)}
 
 
 @pytest.mark.parametrize("storage", ["FP32", "FP16S", "FP16C"])

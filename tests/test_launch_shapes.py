@@ -75,6 +75,20 @@ int main() {
     printf("biglin %u %u %u %d %d %llu\n", d[0], d[1], d[2], (int)use_linear_grid(kl, d[0] + 1u, 4, n, 19),
            (int)use_linear_grid(kl, d[0], 4, n, 19), (unsigned long long)linear_blocks(n, true));
   }
+  // two cells per lane: Nx/2 lanes per row, rows on lines (Nx a multiple of 64), also the
+  // lattices above 4.29 G cells
+  for (unsigned nx : {64u, 128u, 256u, 512u, 1536u, 4096u}) {
+    LaunchShape p = pair_shape(nx, 7, 5);
+    printf("pair %u 7 5 %u %u %u %u %u %d\n", nx, p.block.x, p.grid.x, p.grid.y, p.grid.z, p.xchunks, (int)p.pairs);
+  }
+  for (const auto& d : big) {
+    LaunchShape p = pair_shape(d[0], d[1], d[2]);
+    printf("pair %u %u %u %u %u %u %u %u %d\n", d[0], d[1], d[2], p.block.x, p.grid.x, p.grid.y, p.grid.z, p.xchunks, (int)p.pairs);
+  }
+  printf("tunepairs %d %d %d %d %d %d %d %d\n", (int)tune_pairs(19, ST_FP16S, false, 0u), (int)tune_pairs(19, ST_FP16C, false, EXT_EQUILIBRIUM_BOUNDARIES),
+         (int)tune_pairs(19, ST_FP16S, false, EXT_EQUILIBRIUM_BOUNDARIES | EXT_FORCE_FIELD), (int)tune_pairs(19, ST_FP32, false, 0u),
+         (int)tune_pairs(19, ST_FP16S, true, 0u), (int)tune_pairs(27, ST_FP16S, false, 0u), (int)tune_pairs(19, ST_FP16S, false, EXT_VOLUME_FORCE),
+         (int)tune_pairs(19, ST_FP16C, false, EXT_ALL));
   printf("fits %d %d %d\n", (int)grid_fits(16777215ull, 256u), (int)grid_fits(16777216ull, 256u), (int)grid_fits(4294967295ull, 1u));
   // TRT antisymmetric rate as the host forms it, for 2000 rates across (0, 2)
   for (int k = 1; k <= 2000; k++) {
@@ -115,7 +129,7 @@ def launch_facts(tmp_path_factory):
             lines.setdefault("trt", []).append((int(p[1], 16), int(p[2], 16)))
         elif p[0] == "segment":
             lines[("segment", int(p[1]))] = tuple(int(v) for v in p[2:])
-        elif p[0] in ("big", "biglin"):
+        elif p[0] in ("big", "biglin", "pair", "tunepairs"):
             lines.setdefault(p[0], []).append(tuple(int(v) for v in p[1:]))
         elif p[0] == "fits":
             lines["fits"] = tuple(int(v) for v in p[1:])
@@ -168,7 +182,7 @@ def test_grid_follows_row_alignment(launch_facts):
     blocks of 256 consecutive cells cover the lattice exactly once."""
     _, _, _, lines = launch_facts
     for key, v in lines.items():
-        if key in ("trt", "big", "biglin", "fits"):
+        if key in ("trt", "big", "biglin", "fits", "pair", "tunepairs"):
             continue
         if isinstance(key, tuple) and key[0] == "linear":
             n, xcd = key[1], key[2]
@@ -223,3 +237,25 @@ def test_large_lattices_stay_within_the_launch_limit(launch_facts):
     for Nx, Ny, Nz, off_lines, large_fp32, blocks in lines["biglin"]:
         fits = blocks * 256 <= limit
         assert off_lines == int(fits) and large_fp32 == int(fits), (Nx, Ny, Nz)
+
+
+def test_pair_grid_covers_every_lane(launch_facts):
+    """k_stream_collide_pair: Nx/2 lanes per row (two cells each), blocks of 64/128/256
+    lanes, every chunk of every row once (XCD-mapped when a row takes several blocks, the
+    3-D row grid above the launch limit), within 2^32-1 work-items; selected only for the
+    D3Q19 SRT 16-bit kernels without extensions, with EQUILIBRIUM_BOUNDARIES and with
+    EQUILIBRIUM_BOUNDARIES + FORCE_FIELD."""
+    lines = launch_facts[3]
+    limit = 2 ** 32 - 1
+    assert len(lines["pair"]) == 12
+    for Nx, Ny, Nz, bx, gx, gy, gz, chunks, pairs in lines["pair"]:
+        lanes = Nx // 2
+        row_chunks = -(-lanes // bx)
+        assert pairs == 1 and Nx % 64 == 0 and bx in (64, 128, 256)
+        assert row_chunks * bx >= lanes and (row_chunks - 1) * bx < lanes
+        assert gx * bx <= limit and gy <= 65536 and gz <= 65536
+        if chunks:
+            assert (gy, gz) == (1, 1) and chunks == row_chunks and gx == -(-(Ny * Nz) // 8) * 8 * chunks
+        else:
+            assert (gx, gy, gz) == (row_chunks, Ny, Nz)
+    assert lines["tunepairs"] == [(1, 1, 1, 0, 0, 0, 0, 0)]
