@@ -132,8 +132,9 @@ constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIU
 //   (plain and porous, 384x192x192 to 768x384x192) and 5-10 % for FP16C (512x256x256,
 //   plain D3Q19 and the D3Q27 TRT EQUILIBRIUM_BOUNDARIES + FORCE_FIELD kernel);
 // - one block per row (up to 768 threads) measured no better than chunks of 128/256;
-// - two or four cells per lane measured no better than one for any variant once all
-//   loads are issued ahead of the arithmetic.
+// - two or four cells per lane, each with accesses of its own, measured no better than
+//   one for any variant once all loads are issued ahead of the arithmetic; two cells that
+//   share 32-bit words are tune_pairs' kernel.
 constexpr uint32_t tune_waves_cap(int Q, uint32_t ST, uint32_t EXT) {
 	if(ST != ST_FP32) return 0u;
 	if(EXT == 0u) return 3u;
