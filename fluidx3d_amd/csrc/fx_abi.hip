@@ -454,8 +454,8 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 	if(fx::use_linear_grid(*ctx->k, cfg->Nx, eb, N, cfg->velocity_set)) {
 		ctx->shape_sc = fx::linear_shape(N, ctx->k->stream_collide_linear_xcd);
 		ctx->shape_adj = fx::linear_shape(N, false);
-	} else if(ctx->k->stream_collide_pairs && fx::rows_on_lines(cfg->Nx, eb)) {
-		ctx->shape_sc = fx::pair_shape(cfg->Nx, cfg->Ny, cfg->Nz); // Nx is a multiple of 64 here
+	} else if(ctx->k->stream_collide_pairs && fx::pair_rows(cfg->Nx)) {
+		ctx->shape_sc = fx::pair_shape(cfg->Nx, cfg->Ny, cfg->Nz);
 	}
 	const uint32_t cap = ctx->k->stream_collide_waves_cap;
 	hipError_t e = hipSetDevice(device);

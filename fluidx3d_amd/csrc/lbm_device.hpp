@@ -795,8 +795,8 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false, bool LIN 
 __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
 	stream_collide_cell<Q, ST, TRT, EXT, false, XCD, LIN>(a, nullptr);
 }
-// stream_collide with two adjacent cells per lane, for the 16-bit formats on rows that
-// start on 128-byte lines (Nx a multiple of 64).  Lane (chunk, y, z) takes cells
+// stream_collide with two adjacent cells per lane, for the 16-bit formats on rows of whole
+// waves of lanes (Nx a multiple of 128, pair_rows).  Lane (chunk, y, z) takes cells
 // xa = 2*(chunk*blockDim.x + lane) and xa+1.  Each unshifted slot and each slot shifted
 // only in y/z is one 32-bit word holding both cells' codes (low half: cell xa), so those
 // accesses are 4 bytes per lane instead of 2; the slots shifted in x take one 16-bit
@@ -816,7 +816,9 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
 // EQUILIBRIUM_BOUNDARIES +3.9 %; with 1/64 of the cells solid and scattered, +13 to
 // +19 % (a solid lane no longer leaves a hole in every access of its wave).  The TRT
 // collisions hold 73 VGPRs and gained nothing (FP16C -4 %), D3Q27 needs 133-150 VGPRs
-// (-7 to -25 %); those keep one cell per lane (kernels_for, stream_collide_pairs).
+// (-7 to -25 %); those keep one cell per lane (kernels_for, stream_collide_pairs).  Passing
+// the x-shifted halves between lanes instead (aligned words, ds_bpermute, single-lane
+// accesses at the wave's edges) was bit-exact and 10 % slower.
 typedef __attribute__((address_space(1))) uint16_t gcode;
 typedef __attribute__((address_space(1))) uint32_t gword;
 // Wave-uniform base of a slot row, in scalar registers and in the global address space

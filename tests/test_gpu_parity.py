@@ -74,7 +74,12 @@ def compare_state(lbm, ref, oracle, storage, tag):
     assert bad.size == 0, f"{tag}: {bad.size} DDF values differ, first slot/cell {divmod(int(bad[0]), lbm.N)}: gpu {g[bad[0]]!r} oracle {o[bad[0]]!r}"
 
 
-GRIDS = {"20x12x10": (20, 12, 10), "150x6x5": (150, 6, 5), "520x3x3": (520, 3, 3), "64x5x4": (64, 5, 4), "512x4x3": (512, 4, 3)}
+# 20x12x10, 150x6x5, 520x3x3: rows off the 128-byte lines (cell-linear grid for 16-bit, and
+# for FP32 where Nx*4 misses them); 64x5x4: rows on lines, one block; 512x4x3: one block per
+# row, and two cells per lane for the D3Q19 SRT 16-bit kernels; 1024x3x2: rows of several
+# blocks on the XCD-mapped grid, with both kernel forms
+GRIDS = {"20x12x10": (20, 12, 10), "150x6x5": (150, 6, 5), "520x3x3": (520, 3, 3), "64x5x4": (64, 5, 4), "512x4x3": (512, 4, 3),
+         "1024x3x2": (1024, 3, 2)}
 
 
 @pytest.mark.parametrize("grid", GRIDS)

@@ -75,9 +75,10 @@ int main() {
     printf("biglin %u %u %u %d %d %llu\n", d[0], d[1], d[2], (int)use_linear_grid(kl, d[0] + 1u, 4, n, 19),
            (int)use_linear_grid(kl, d[0], 4, n, 19), (unsigned long long)linear_blocks(n, true));
   }
-  // two cells per lane: Nx/2 lanes per row, rows on lines (Nx a multiple of 64), also the
-  // lattices above 4.29 G cells
-  for (unsigned nx : {64u, 128u, 256u, 512u, 1536u, 4096u}) {
+  // two cells per lane: Nx/2 lanes per row filling whole waves (Nx a multiple of 128), also
+  // the lattices above 4.29 G cells
+  for (unsigned nx : {64u, 128u, 192u, 256u, 512u, 1536u, 4096u}) printf("pairrows %u %d\n", nx, (int)pair_rows(nx));
+  for (unsigned nx : {128u, 256u, 384u, 512u, 1536u, 4096u}) {
     LaunchShape p = pair_shape(nx, 7, 5);
     printf("pair %u 7 5 %u %u %u %u %u %d\n", nx, p.block.x, p.grid.x, p.grid.y, p.grid.z, p.xchunks, (int)p.pairs);
   }
@@ -129,7 +130,7 @@ def launch_facts(tmp_path_factory):
             lines.setdefault("trt", []).append((int(p[1], 16), int(p[2], 16)))
         elif p[0] == "segment":
             lines[("segment", int(p[1]))] = tuple(int(v) for v in p[2:])
-        elif p[0] in ("big", "biglin", "pair", "tunepairs"):
+        elif p[0] in ("big", "biglin", "pair", "tunepairs", "pairrows"):
             lines.setdefault(p[0], []).append(tuple(int(v) for v in p[1:]))
         elif p[0] == "fits":
             lines["fits"] = tuple(int(v) for v in p[1:])
@@ -182,7 +183,7 @@ def test_grid_follows_row_alignment(launch_facts):
     blocks of 256 consecutive cells cover the lattice exactly once."""
     _, _, _, lines = launch_facts
     for key, v in lines.items():
-        if key in ("trt", "big", "biglin", "fits", "pair", "tunepairs"):
+        if key in ("trt", "big", "biglin", "fits", "pair", "tunepairs", "pairrows"):
             continue
         if isinstance(key, tuple) and key[0] == "linear":
             n, xcd = key[1], key[2]
@@ -251,7 +252,7 @@ def test_pair_grid_covers_every_lane(launch_facts):
     for Nx, Ny, Nz, bx, gx, gy, gz, chunks, pairs in lines["pair"]:
         lanes = Nx // 2
         row_chunks = -(-lanes // bx)
-        assert pairs == 1 and Nx % 64 == 0 and bx in (64, 128, 256)
+        assert pairs == 1 and Nx % 128 == 0 and bx in (64, 128, 256) and lanes % 64 == 0
         assert row_chunks * bx >= lanes and (row_chunks - 1) * bx < lanes
         assert gx * bx <= limit and gy <= 65536 and gz <= 65536
         if chunks:
@@ -259,3 +260,4 @@ def test_pair_grid_covers_every_lane(launch_facts):
         else:
             assert (gx, gy, gz) == (row_chunks, Ny, Nz)
     assert lines["tunepairs"] == [(1, 1, 1, 0, 0, 0, 0, 0)]
+    assert dict((r[0], r[1]) for r in lines["pairrows"]) == {64: 0, 128: 1, 192: 0, 256: 1, 512: 1, 1536: 1, 4096: 1}
