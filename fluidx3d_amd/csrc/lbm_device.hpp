@@ -808,9 +808,12 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
 // first, and the cell's memory is left as it was.  Same results as k_stream_collide, bit
 // for bit.
 //
-// The 16-bit kernels keep the texture address unit 81 % (FP16S) and 91 % (FP16C) busy at
-// about 10 cycles per wave access whether it moves 2 or 4 bytes per lane, so fewer, wider
-// accesses raise the rate.  Measured at 256^3 against k_stream_collide, same process,
+// The one-cell 16-bit kernels keep the texture address unit 81 % (FP16S) and 91 % (FP16C)
+// busy at about 10 cycles per wave access, 2 bytes per lane.  Here the 32-bit words cost
+// about 13 cycles and the x-shifted 2-byte accesses, two elements apart, about 25, so the
+// address work per cell stays the same, while the scalar instructions per cell fall from
+// 3.2 to 2.0 (one flag load and one set of row offsets per two cells), and the gain is
+// small.  Measured at 256^3 against k_stream_collide, same process,
 // D3Q19 SRT: FP16S none +2.5 %, EQUILIBRIUM_BOUNDARIES +5.0 %, + FORCE_FIELD +5.9 %;
 // FP16C none +1.2 %, EQUILIBRIUM_BOUNDARIES +5.4 %; 1536x768x768 FP16S
 // EQUILIBRIUM_BOUNDARIES +3.9 %; with 1/64 of the cells solid and scattered, +13 to
