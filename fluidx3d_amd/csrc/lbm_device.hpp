@@ -109,6 +109,7 @@ template<> struct Storage<ST_FP16S> {
 	typedef uint16_t E;
 	__device__ static __forceinline__ float decode(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h) * 3.0517578E-5f; }
 	__device__ static __forceinline__ uint16_t encode(float x) { return __builtin_bit_cast(uint16_t, (_Float16)(x * 32768.0f)); }
+	__device__ static __forceinline__ uint32_t encode_word(float x) { return encode(x); }
 	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
 	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(raw(p, o)); }
 	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { __builtin_nontemporal_store(encode(x), elem(p, o)); }
@@ -139,14 +140,20 @@ template<> struct Storage<ST_FP16C> {
 	// |x|*2^25 = k - 1/2 into k (ties to even) and keeps every value below an integer below
 	// it (checked on all 2^32 floats; the addend 1/2 itself fails at |x|*2^25 = 1/2 - 2^-25).
 	// A NaN fails the comparison and takes the normal path, as in the reference.
-	__device__ static __forceinline__ uint16_t encode(float x) {
+	// The code in a 32-bit word, not truncated to 16 bits: it is below 2^16 for every input
+	// (the normal code is a 15-bit field and the denormal one is taken only below 2^-14,
+	// where it is below 2^11), but the compiler cannot see that and would mask the denormal
+	// branch to 16 bits, one VALU operation per DDF; 16-bit stores and the two-cell packing
+	// drop the upper bits themselves.
+	__device__ static __forceinline__ uint32_t encode_word(float x) {
 		const uint32_t b = __float_as_uint(x) + 0x00000800u;
 		const uint32_t normal = (b >> 12) & 0x7FFFu;
 		const float ax = fabsf(x);
 		const uint32_t denormal = __float2uint_rz(__builtin_fmaf(ax, 0x1p25f, 0x1.fffffep-2f));
 		const uint32_t mag = ax < __uint_as_float(0x387FF800u) ? denormal : normal;
-		return (uint16_t)(((b >> 16) & 0x8000u) | mag);
+		return ((b >> 16) & 0x8000u) | mag;
 	}
+	__device__ static __forceinline__ uint16_t encode(float x) { return (uint16_t)encode_word(x); }
 	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
 	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(raw(p, o)); }
 	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { __builtin_nontemporal_store(encode(x), elem(p, o)); }
@@ -872,7 +879,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 		for(int i = 0; i < Q; i++) f[i] = S::decode((uint16_t)P[i]);
 		collide_cell<Q, TRT, EXT, false>(a, n, flA & TYPE_BO, f, nullptr);
 #pragma unroll
-		for(int i = 0; i < Q; i++) P[i] = (P[i] & 0xFFFF0000u) | (uint32_t)S::encode(f[i]);
+		for(int i = 0; i < Q; i++) P[i] = (P[i] & 0xFFFF0000u) | S::encode_word(f[i]);
 	} else {
 #pragma unroll
 		for(int i = 1; i < Q; i += 2) { const uint32_t t = (P[i] ^ P[i + 1]) & 0x0000FFFFu; P[i] ^= t; P[i + 1] ^= t; }
@@ -883,7 +890,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 		for(int i = 0; i < Q; i++) f[i] = S::decode((uint16_t)(P[i] >> 16));
 		collide_cell<Q, TRT, EXT, false>(a, n + 1u, flB & TYPE_BO, f, nullptr);
 #pragma unroll
-		for(int i = 0; i < Q; i++) P[i] = (P[i] & 0x0000FFFFu) | ((uint32_t)S::encode(f[i]) << 16);
+		for(int i = 0; i < Q; i++) P[i] = __builtin_amdgcn_perm(S::encode_word(f[i]), P[i], 0x05040100u); // (code << 16) | (P & 0xFFFF)
 	} else {
 #pragma unroll
 		for(int i = 1; i < Q; i += 2) { const uint32_t t = (P[i] ^ P[i + 1]) & 0xFFFF0000u; P[i] ^= t; P[i + 1] ^= t; }
