@@ -456,6 +456,8 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 		ctx->shape_adj = fx::linear_shape(N, false);
 	} else if(ctx->k->stream_collide_pairs && fx::pair_rows(cfg->Nx)) {
 		ctx->shape_sc = fx::pair_shape(cfg->Nx, cfg->Ny, cfg->Nz);
+	} else if(ctx->k->stream_collide_offsets32 && N * eb <= 0xFFFFFFFFull) {
+		ctx->shape_sc.off32 = true; // every slot below 4 GiB: 32-bit byte offsets
 	}
 	const uint32_t cap = ctx->k->stream_collide_waves_cap;
 	hipError_t e = hipSetDevice(device);

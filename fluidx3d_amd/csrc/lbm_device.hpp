@@ -360,11 +360,13 @@ __device__ __forceinline__ Site make_site(uint32_t x, uint32_t y, uint32_t z, ui
 // unshifted slots; 69-71 instead of 79 VGPRs for the plain kernels, measured +0.4 to
 // +7.6 % on 14 of the 16 D3Q19 16-bit variants at 200^3, D3Q27 -4.8 to +18 % (11 of 16
 // up); FP32 -2.3 to +1 %, not taken.
+// O32: the same 32-bit offsets on the row grid (the row grid's access rule), for lattices
+// whose slots are below 4 GiB; tune_offsets32 selects the kernels that gain from it.
 template<int Q, uint32_t ST> constexpr bool slot_offsets32(bool SC) { return SC && ST != ST_FP32; }
-template<int Q, uint32_t ST, bool U = false, bool SC = false>
+template<int Q, uint32_t ST, bool U = false, bool SC = false, bool O32 = false>
 __device__ __forceinline__ void load_raw(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, typename Storage<ST>::E* r) {
 	typedef Storage<ST> S;
-	if constexpr(slot_offsets32<Q, ST>(SC)) {
+	if constexpr(O32 || slot_offsets32<Q, ST>(SC)) {
 		const uint32_t c = (uint32_t)s.R[4] + s.x; // the cell's index
 		r[0] = S::raw(fi, c);
 #pragma unroll
@@ -372,7 +374,9 @@ __device__ __forceinline__ void load_raw(const typename Storage<ST>::E* fi, uint
 			const uint64_t sa = odd ? (uint64_t)i : (uint64_t)(i + 1);
 			const uint64_t sb = odd ? (uint64_t)(i + 1) : (uint64_t)i;
 			r[i] = S::raw(fi + sa * N, c);
-			r[i + 1] = S::raw_cached(fi + sb * N, (uint32_t)s.row(VS<Q>::cy[i], VS<Q>::cz[i]) + s.X[VS<Q>::cx[i] + 1]);
+			const uint32_t o = (uint32_t)s.row(VS<Q>::cy[i], VS<Q>::cz[i]) + s.X[VS<Q>::cx[i] + 1];
+			if(SC || VS<Q>::cx[i] != 0) r[i + 1] = S::raw_cached(fi + sb * N, o);
+			else r[i + 1] = S::raw(fi + sb * N, o);
 		}
 		return;
 	}
@@ -389,24 +393,26 @@ __device__ __forceinline__ void load_raw(const typename Storage<ST>::E* fi, uint
 		else r[i + 1] = VS<Q>::cx[i] != 0 ? S::template raw_shifted<Q>(pb, xb) : S::raw(pb, xb);
 	}
 }
-template<int Q, uint32_t ST, bool U = false, bool SC = false>
+template<int Q, uint32_t ST, bool U = false, bool SC = false, bool O32 = false>
 __device__ __forceinline__ void load_f(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, float* f) {
 	typename Storage<ST>::E r[Q];
-	load_raw<Q, ST, U, SC>(fi, N, s, odd, r);
+	load_raw<Q, ST, U, SC, O32>(fi, N, s, odd, r);
 #pragma unroll
 	for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
 }
-template<int Q, uint32_t ST, bool U = false, bool SC = false>
+template<int Q, uint32_t ST, bool U = false, bool SC = false, bool O32 = false>
 __device__ __forceinline__ void store_f(typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, const float* f) {
 	typedef Storage<ST> S;
-	if constexpr(slot_offsets32<Q, ST>(SC)) {
+	if constexpr(O32 || slot_offsets32<Q, ST>(SC)) {
 		const uint32_t c = (uint32_t)s.R[4] + s.x;
 		S::store(fi, c, f[0]);
 #pragma unroll
 		for(int i = 1; i < Q; i += 2) {
 			const uint64_t sa = odd ? (uint64_t)i : (uint64_t)(i + 1);
 			const uint64_t sb = odd ? (uint64_t)(i + 1) : (uint64_t)i;
-			S::store_cached(fi + sb * N, (uint32_t)s.row(VS<Q>::cy[i], VS<Q>::cz[i]) + s.X[VS<Q>::cx[i] + 1], f[i]);
+			const uint32_t o = (uint32_t)s.row(VS<Q>::cy[i], VS<Q>::cz[i]) + s.X[VS<Q>::cx[i] + 1];
+			if(SC || VS<Q>::cx[i] != 0) S::store_cached(fi + sb * N, o, f[i]);
+			else S::store(fi + sb * N, o, f[i]);
 			S::store(fi + sa * N, c, f[i + 1]);
 		}
 		return;
@@ -763,7 +769,7 @@ template<bool XCD, bool LIN> __device__ __forceinline__ bool cell_coords(const L
 	}
 }
 // One lane per cell.
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD, bool XCD, bool LIN>
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD, bool XCD, bool LIN, bool O32 = false>
 __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 	a.ext &= EXT;
 	if constexpr(constant_vf_mask<TRT>(Q, ST, EXT, LIN)) a.ext = EXT_VOLUME_FORCE;
@@ -783,24 +789,24 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 		// convert each 16-bit value right after its load and wait for it, one at a time
 		E r[Q];
 		if constexpr(load_priority<TRT>(EXT)) __builtin_amdgcn_s_setprio(1);
-		load_raw<Q, ST, U, LIN>(fi, a.N, s, a.odd, r);
+		load_raw<Q, ST, U, LIN, O32>(fi, a.N, s, a.odd, r);
 		__builtin_amdgcn_sched_barrier(0);
 		if constexpr(load_priority<TRT>(EXT)) __builtin_amdgcn_s_setprio(0);
 #pragma unroll
 		for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
 	} else {
-		load_f<Q, ST, U, LIN>(fi, a.N, s, a.odd, f);
+		load_f<Q, ST, U, LIN, O32>(fi, a.N, s, a.odd, f);
 	}
 	if((a.ext & EXT_MOVING_BOUNDARIES) && bo == TYPE_MS) moving_boundaries<Q>(a, s, f);
 	collide_cell<Q, TRT, EXT, RECORD>(a, n, bo, f, moments);
-	store_f<Q, ST, U, LIN>(fi, a.N, s, a.odd, f);
+	store_f<Q, ST, U, LIN, O32>(fi, a.N, s, a.odd, f);
 }
 // XCD: the kernel may run on the 1-D XCD-mapped grid (a.xchunks > 0); false compiles the
 // mapping out.  LIN: the cell-linear grid (cell_coords), for lattices whose rows do not
 // start on 128-byte lines.
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false, bool LIN = false>
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false, bool LIN = false, bool O32 = false>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
-	stream_collide_cell<Q, ST, TRT, EXT, false, XCD, LIN>(a, nullptr);
+	stream_collide_cell<Q, ST, TRT, EXT, false, XCD, LIN, O32>(a, nullptr);
 }
 // stream_collide with two adjacent cells per lane, for the 16-bit formats on rows of whole
 // waves of lanes (Nx a multiple of 128, pair_rows).  Lane (chunk, y, z) takes cells

@@ -11,6 +11,7 @@ struct LaunchShape {
 	uint32_t xchunks = 0; // 1-D XCD-mapped grid with this many x chunks per row (see block_coords)
 	bool lin = false;     // cell-linear grid (cell_coords in lbm_device.hpp), see linear_shape
 	bool pairs = false;   // two cells per lane (k_stream_collide_pair), see pair_shape
+	bool off32 = false;   // row grid with 32-bit slot offsets (O32 of load_raw/store_f)
 };
 // A launch holds at most 2^32-1 work-items along each grid dimension (grid size times
 // block size); lattices of more than ~4.29 G cells (a D3Q19 FP16 lattice of up to ~5.2 G
@@ -120,6 +121,7 @@ struct LbmKernels {
 	bool stream_collide_linear_large; // cell-linear grid for DDF arrays above LARGE_DDF_BYTES
 	bool stream_collide_linear_xcd;   // XCD-aware segment order on the cell-linear grid
 	bool stream_collide_pairs;        // k_stream_collide_pair on rows that start on lines
+	bool stream_collide_offsets32;    // 32-bit slot offsets on the row grid (slots below 4 GiB)
 };
 
 constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY |
@@ -148,6 +150,19 @@ constexpr uint32_t tune_waves_cap(int Q, uint32_t ST, uint32_t EXT) {
 }
 constexpr bool tune_xcd_map(int Q, uint32_t ST, uint32_t EXT) {
 	return true;
+}
+// 32-bit slot offsets on the row grid (load_raw's O32: one base per slot in scalar registers
+// and one 32-bit offset per lane instead of 64-bit per-lane addresses): the D3Q27 16-bit
+// kernels spend about 45 of their ~850 VALU instructions per wave on 64-bit address
+// arithmetic, and the C3-type kernel (TRT FP16C, EQUILIBRIUM_BOUNDARIES + FORCE_FIELD) keeps
+// the VALU 99 % busy.  Measured at 256^3 against 64-bit addressing, same process, bit-exact:
+// SRT none +4.6 % (FP16S) / +2.3 % (FP16C), EQUILIBRIUM_BOUNDARIES +3.7 / +2.2,
+// VOLUME_FORCE +2.9 / -0.7, + FORCE_FIELD +2.9 / +1.6, porous +1.3 / +2.1; TRT
+// EQUILIBRIUM_BOUNDARIES +2.6 / +1.3, + FORCE_FIELD +2.7 / +0.7 (+2.5 % for FP16C at
+// 512x256x256), porous +0.2 / +2.4; TRT none -3.3 / -4.0 and VOLUME_FORCE -3.6 / -4.5 and the
+// all-extension kernels -4.0 to +0.5 % keep 64-bit addresses.
+constexpr bool tune_offsets32(int Q, uint32_t ST, bool TRT, uint32_t EXT) {
+	return Q == 27 && ST != ST_FP32 && EXT != EXT_ALL && (!TRT || (EXT != 0u && EXT != EXT_VOLUME_FORCE));
 }
 // Two cells per lane (k_stream_collide_pair, where its measurements are): the D3Q19 SRT
 // 16-bit kernels without extensions, with EQUILIBRIUM_BOUNDARIES and with
@@ -188,6 +203,9 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 		if constexpr(pairs) {
 			if(s.pairs) { k_stream_collide_pair<Q, ST, TRT, EXT><<<s.grid, s.block, s.lds, st>>>(b); return; }
 		}
+		if constexpr(offsets32) {
+			if(s.off32) { k_stream_collide<Q, ST, TRT, EXT, xcd_map, false, true><<<s.grid, s.block, s.lds, st>>>(b); return; }
+		}
 		if(s.lin) k_stream_collide<Q, ST, TRT, EXT, linear_xcd, true><<<s.grid, s.block, s.lds, st>>>(b);
 		else k_stream_collide<Q, ST, TRT, EXT, xcd_map, false><<<s.grid, s.block, s.lds, st>>>(b);
 	}
@@ -218,6 +236,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static constexpr bool xcd_map = tune_xcd_map(Q, ST, EXT);
 	static constexpr bool linear_xcd = tune_linear_xcd(Q, ST, EXT);
 	static constexpr bool pairs = tune_pairs(Q, ST, TRT, EXT);
+	static constexpr bool offsets32 = tune_offsets32(Q, ST, TRT, EXT);
 	template<int AXIS> static void transfer_axis(void* fi, void* bp, void* bm, uint32_t Nx, uint32_t Ny, uint32_t Nz, bool insert, uint32_t odd, hipStream_t st) {
 		typedef typename Storage<ST>::E E;
 		const uint64_t A = AXIS == 0 ? (uint64_t)Ny * Nz : AXIS == 1 ? (uint64_t)Nx * Nz : (uint64_t)Nx * Ny;
@@ -233,7 +252,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static const LbmKernels* table() {
 		static const LbmKernels k = {initialize, stream_collide, update_fields, update_force_field, mass_partial, waves_cap,
 		                             stream_collide_record, adjoint_step, objective_partial, xcd_map, transfer_fi,
-		                             update_moving_boundaries, tune_linear_large(Q, ST, EXT), linear_xcd, pairs};
+		                             update_moving_boundaries, tune_linear_large(Q, ST, EXT), linear_xcd, pairs, offsets32};
 		return &k;
 	}
 };

@@ -90,6 +90,15 @@ int main() {
          (int)tune_pairs(19, ST_FP16S, false, EXT_EQUILIBRIUM_BOUNDARIES | EXT_FORCE_FIELD), (int)tune_pairs(19, ST_FP32, false, 0u),
          (int)tune_pairs(19, ST_FP16S, true, 0u), (int)tune_pairs(27, ST_FP16S, false, 0u), (int)tune_pairs(19, ST_FP16S, false, EXT_VOLUME_FORCE),
          (int)tune_pairs(19, ST_FP16C, false, EXT_ALL));
+  {
+    constexpr uint32_t E = EXT_EQUILIBRIUM_BOUNDARIES, VF = EXT_VOLUME_FORCE, FF = EXT_FORCE_FIELD, POR = EXT_VOLUME_FORCE | EXT_POROSITY | EXT_PRECISE_DENSITY;
+    const uint32_t exts[6] = {0u, E, VF, E | FF, POR, EXT_ALL};
+    for (int st = 0; st < 3; st++) for (int trt = 0; trt < 2; trt++) for (int q : {19, 27}) {
+      printf("off32 %d %d %d", q, st, trt);
+      for (uint32_t e : exts) printf(" %d", (int)tune_offsets32(q, (uint32_t)st, trt != 0, e));
+      printf("\n");
+    }
+  }
   printf("fits %d %d %d\n", (int)grid_fits(16777215ull, 256u), (int)grid_fits(16777216ull, 256u), (int)grid_fits(4294967295ull, 1u));
   // TRT antisymmetric rate as the host forms it, for 2000 rates across (0, 2)
   for (int k = 1; k <= 2000; k++) {
@@ -130,7 +139,7 @@ def launch_facts(tmp_path_factory):
             lines.setdefault("trt", []).append((int(p[1], 16), int(p[2], 16)))
         elif p[0] == "segment":
             lines[("segment", int(p[1]))] = tuple(int(v) for v in p[2:])
-        elif p[0] in ("big", "biglin", "pair", "tunepairs", "pairrows"):
+        elif p[0] in ("big", "biglin", "pair", "tunepairs", "pairrows", "off32"):
             lines.setdefault(p[0], []).append(tuple(int(v) for v in p[1:]))
         elif p[0] == "fits":
             lines["fits"] = tuple(int(v) for v in p[1:])
@@ -183,7 +192,7 @@ def test_grid_follows_row_alignment(launch_facts):
     blocks of 256 consecutive cells cover the lattice exactly once."""
     _, _, _, lines = launch_facts
     for key, v in lines.items():
-        if key in ("trt", "big", "biglin", "fits", "pair", "tunepairs", "pairrows"):
+        if key in ("trt", "big", "biglin", "fits", "pair", "tunepairs", "pairrows", "off32"):
             continue
         if isinstance(key, tuple) and key[0] == "linear":
             n, xcd = key[1], key[2]
@@ -261,3 +270,18 @@ def test_pair_grid_covers_every_lane(launch_facts):
             assert (gx, gy, gz) == (row_chunks, Ny, Nz)
     assert lines["tunepairs"] == [(1, 1, 1, 0, 0, 0, 0, 0)]
     assert dict((r[0], r[1]) for r in lines["pairrows"]) == {64: 0, 128: 1, 192: 0, 256: 1, 512: 1, 1536: 1, 4096: 1}
+
+
+def test_offsets32_selection(launch_facts):
+    """32-bit slot offsets on the row grid only for the D3Q27 16-bit kernels where they
+    measured faster: every SRT subset but the all-extension one; TRT with
+    EQUILIBRIUM_BOUNDARIES (+ FORCE_FIELD) and the porous subset."""
+    table = {(q, st, trt): flags for q, st, trt, *flags in launch_facts[3]["off32"]}
+    assert len(table) == 12
+    for (q, st, trt), flags in table.items():
+        if q == 19 or st == 0:
+            assert flags == [0, 0, 0, 0, 0, 0], (q, st, trt)
+        elif trt == 0:
+            assert flags == [1, 1, 1, 1, 1, 0], (q, st, trt)   # none, E, VF, E+FF, porous | all
+        else:
+            assert flags == [0, 1, 0, 1, 1, 0], (q, st, trt)
