@@ -160,9 +160,15 @@ constexpr bool tune_xcd_map(int Q, uint32_t ST, uint32_t EXT) {
 // VOLUME_FORCE +2.9 / -0.7, + FORCE_FIELD +2.9 / +1.6, porous +1.3 / +2.1; TRT
 // EQUILIBRIUM_BOUNDARIES +2.6 / +1.3, + FORCE_FIELD +2.7 / +0.7 (+2.5 % for FP16C at
 // 512x256x256), porous +0.2 / +2.4; TRT none -3.3 / -4.0 and VOLUME_FORCE -3.6 / -4.5 and the
-// all-extension kernels -4.0 to +0.5 % keep 64-bit addresses.
+// all-extension kernels -4.0 to +0.5 % keep 64-bit addresses.  D3Q19 16-bit kernels that do
+// not take two cells per lane: TRT none +5.2 / +1.4 %, EQUILIBRIUM_BOUNDARIES +4.3 / +0.5,
+// + FORCE_FIELD +3.4 / -0.2, porous +1.6 / +4.5 take 32-bit offsets; TRT VOLUME_FORCE
+// (-1.0 / -4.4), SRT VOLUME_FORCE and porous (-16 to +2 %) and the all-extension kernels
+// (-15 to -5 %) keep their addressing (uniform row bases where uniform_rows says so).
 constexpr bool tune_offsets32(int Q, uint32_t ST, bool TRT, uint32_t EXT) {
-	return Q == 27 && ST != ST_FP32 && EXT != EXT_ALL && (!TRT || (EXT != 0u && EXT != EXT_VOLUME_FORCE));
+	if(ST == ST_FP32 || EXT == EXT_ALL) return false;
+	if(Q == 27) return !TRT || (EXT != 0u && EXT != EXT_VOLUME_FORCE);
+	return TRT && EXT != EXT_VOLUME_FORCE;
 }
 // Two cells per lane (k_stream_collide_pair, where its measurements are): the D3Q19 SRT
 // 16-bit kernels without extensions, with EQUILIBRIUM_BOUNDARIES and with

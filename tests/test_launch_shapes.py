@@ -273,15 +273,21 @@ def test_pair_grid_covers_every_lane(launch_facts):
 
 
 def test_offsets32_selection(launch_facts):
-    """32-bit slot offsets on the row grid only for the D3Q27 16-bit kernels where they
-    measured faster: every SRT subset but the all-extension one; TRT with
-    EQUILIBRIUM_BOUNDARIES (+ FORCE_FIELD) and the porous subset."""
+    """32-bit slot offsets on the row grid only for the 16-bit kernels where they measured
+    faster: D3Q27 every SRT subset but the all-extension one, TRT with
+    EQUILIBRIUM_BOUNDARIES (+ FORCE_FIELD) and the porous subset; D3Q19 the TRT subsets
+    other than VOLUME_FORCE and the all-extension one (the D3Q19 SRT kernels without
+    extensions or with EQUILIBRIUM_BOUNDARIES (+ FORCE_FIELD) take two cells per lane)."""
     table = {(q, st, trt): flags for q, st, trt, *flags in launch_facts[3]["off32"]}
     assert len(table) == 12
-    for (q, st, trt), flags in table.items():
-        if q == 19 or st == 0:
+    for (q, st, trt), flags in table.items():   # none, E, VF, E+FF, porous, all
+        if st == 0:
             assert flags == [0, 0, 0, 0, 0, 0], (q, st, trt)
-        elif trt == 0:
-            assert flags == [1, 1, 1, 1, 1, 0], (q, st, trt)   # none, E, VF, E+FF, porous | all
-        else:
+        elif q == 27 and trt == 0:
+            assert flags == [1, 1, 1, 1, 1, 0], (q, st, trt)
+        elif q == 27:
             assert flags == [0, 1, 0, 1, 1, 0], (q, st, trt)
+        elif trt == 0:
+            assert flags == [0, 0, 0, 0, 0, 0], (q, st, trt)
+        else:
+            assert flags == [1, 1, 0, 1, 1, 0], (q, st, trt)
