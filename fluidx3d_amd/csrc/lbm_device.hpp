@@ -809,32 +809,31 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
 	stream_collide_cell<Q, ST, TRT, EXT, false, XCD, LIN, O32>(a, nullptr);
 }
 // stream_collide with two adjacent cells per lane, for the 16-bit formats on rows of whole
-// waves of lanes (Nx a multiple of 128, pair_rows).  Lane (chunk, y, z) takes cells
-// xa = 2*(chunk*blockDim.x + lane) and xa+1.  Each unshifted slot and each slot shifted
-// only in y/z is one 32-bit word holding both cells' codes (low half: cell xa), so those
-// accesses are 4 bytes per lane instead of 2; the slots shifted in x take one 16-bit
-// access per cell (xa's neighbour and xa+1's are not one aligned word, and the one at the
-// row's end wraps).  The per-lane words are decoded, collided and re-encoded one cell at a
-// time, so only one cell's floats are live.  A cell that does not collide (solid or gas)
-// writes each of its codes back to where it was loaded from: Esoteric-Pull stores a
-// population to the other location of its pair, so the codes of each pair are swapped
-// first, and the cell's memory is left as it was.  Same results as k_stream_collide, bit
-// for bit.
+// waves of lanes (Nx a multiple of 128, pair_rows).  A wave takes the 128 cells c0 .. c0+127
+// of one row, lane l the cells xa = c0 + 2l and xa+1.  Each unshifted slot and each slot
+// shifted only in y/z is one 32-bit word per lane holding both cells' codes (low half: cell
+// xa), so those accesses are 4 bytes per lane instead of 2.  The slots shifted in x are
+// accessed as two contiguous runs of 64 codes, lane m taking the neighbour of cell c0+m and
+// then of c0+64+m (periodic at the row ends), and the codes move between those lanes and the
+// cells' lanes with ds_bpermute: each run access touches the two lines the 64 codes span,
+// where a per-cell 16-bit access at a stride of two codes (xa's neighbour and xa+1's are not
+// one aligned word) would span three.  The per-lane words are decoded, collided and
+// re-encoded one cell at a time, so only one cell's floats are live.  A cell that does not
+// collide (solid or gas) writes each of its codes back to where it was loaded from:
+// Esoteric-Pull stores a population to the other location of its pair, so the codes of each
+// pair are swapped first, and the cell's memory is left as it was (a wave with no live cell
+// returns at once).  Same results as k_stream_collide, bit for bit.
 //
 // The one-cell 16-bit kernels keep the texture address unit 81 % (FP16S) and 91 % (FP16C)
 // busy at about 10 cycles per wave access, 2 bytes per lane.  Here the 32-bit words cost
-// about 13 cycles and the x-shifted 2-byte accesses, two elements apart, about 25, so the
-// address work per cell stays the same, while the scalar instructions per cell fall from
-// 3.2 to 2.0 (one flag load and one set of row offsets per two cells), and the gain is
-// small.  Measured at 256^3 against k_stream_collide, same process,
-// D3Q19 SRT: FP16S none +2.5 %, EQUILIBRIUM_BOUNDARIES +5.0 %, + FORCE_FIELD +5.9 %;
-// FP16C none +1.2 %, EQUILIBRIUM_BOUNDARIES +5.4 %; 1536x768x768 FP16S
-// EQUILIBRIUM_BOUNDARIES +3.9 %; with 1/64 of the cells solid and scattered, +13 to
-// +19 % (a solid lane no longer leaves a hole in every access of its wave).  The TRT
-// collisions hold 73 VGPRs and gained nothing (FP16C -4 %), D3Q27 needs 133-150 VGPRs
-// (-7 to -25 %); those keep one cell per lane (kernels_for, stream_collide_pairs).  Passing
-// the x-shifted halves between lanes instead (aligned words, ds_bpermute, single-lane
-// accesses at the wave's edges) was bit-exact and 10 % slower.
+// about 13 cycles; with one 16-bit access per cell at the x-shifted slots (about 25 cycles
+// each, two codes apart) the address work per cell stayed that of the one-cell kernel and
+// the gain over it was small (D3Q19 SRT at 256^3: FP16S +2.5 %, FP16C +1.2 %, EQUILIBRIUM_
+// BOUNDARIES +5 %; the scalar instructions per cell fell from 3.2 to 2.0, one flag load and
+// one set of row offsets per two cells); the contiguous runs added +6 to +7 % (FP16S) and
+// +6 % (FP16C) at 256^3 and +6 to +7 % (FP16S) at 1024x128x96, same process, bit-exact.
+// Earlier forms that moved halves between lanes from aligned words needed single-lane
+// accesses at the wave's edges, which cost about as much as full ones (10 % slower).
 typedef __attribute__((address_space(1))) uint16_t gcode;
 typedef __attribute__((address_space(1))) uint32_t gword;
 // Wave-uniform base of a slot row, in scalar registers and in the global address space
@@ -853,19 +852,25 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 	typedef Storage<ST> S;
 	uint32_t chunk, y, z;
 	if(!block_coords<true>(a, chunk, y, z)) return;
-	const uint32_t xa = 2u * (chunk * blockDim.x + threadIdx.x);
-	if(xa >= a.Nx) return;
-	const Site s = make_site(xa, y, z, a.Nx, a.Ny, a.Nz); // X = {xa-1, xa, xa+1}: the first cell
-	const uint32_t XB[3] = {xa, xa + 1u, xa + 2u == a.Nx ? 0u : xa + 2u}; // the second cell
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint32_t c0 = 2u * (chunk * blockDim.x + (threadIdx.x & ~63u)); // the wave's first cell
+	if(c0 >= a.Nx) return; // whole waves: Nx/2 lanes per row, a multiple of 64
+	const uint32_t xa = c0 + 2u * lane;
+	const Site s = make_site(xa, y, z, a.Nx, a.Ny, a.Nz);
 	const uint64_t n = s.n();
 	const uint32_t fl = *(const uint16_t*)(a.flags + n); // n is even: flags[n] low, flags[n+1] high
 	const uint32_t flA = fl & 0xFFu, flB = fl >> 8;
 	const bool liveA = !((flA & TYPE_BO) == TYPE_S || (flA & TYPE_SU) == TYPE_G);
 	const bool liveB = !((flB & TYPE_BO) == TYPE_S || (flB & TYPE_SU) == TYPE_G);
-	if(!liveA && !liveB) return;
+	if(__ballot(liveA || liveB) == 0ull) return; // wave-uniform: every lane takes part in the exchanges
+	// run positions: lane m's cells c0+m and c0+64+m, their x-1 / x+1 neighbours (the row
+	// wraps only at c0+m-1 for c0 = 0 and at c0+65+m for the row's last wave)
+	const uint32_t r1 = c0 + lane, r2 = r1 + 64u;
+	const uint32_t r1m = r1 == 0u ? a.Nx - 1u : r1 - 1u, r2p = r2 + 1u == a.Nx ? 0u : r2 + 1u;
 	uint16_t* fi = (uint16_t*)a.fi;
 	const uint64_t N = a.N, n0 = s.R[4];
-	uint32_t P[Q]; // population i of both cells, as loaded (load_raw's order)
+	uint32_t P[Q]; // population i of both cells, as loaded (load_raw's order): low half cell xa
+	uint32_t X[Q]; // x-shifted slots: the codes of the runs, low half cell c0+m's, high c0+64+m's
 	__builtin_amdgcn_s_setprio(1);
 	P[0] = __builtin_nontemporal_load(word_at(global_row(fi + n0), xa));
 #pragma unroll
@@ -875,10 +880,24 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 		P[i] = __builtin_nontemporal_load(word_at(global_row(fi + sa * N + n0), xa));
 		gcode* pb = global_row(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]));
 		if(VS<Q>::cx[i] == 0) P[i + 1] = __builtin_nontemporal_load(word_at(pb, xa));
-		else P[i + 1] = (uint32_t)*code_at(pb, s.X[VS<Q>::cx[i] + 1]) | ((uint32_t)*code_at(pb, XB[VS<Q>::cx[i] + 1]) << 16);
+		else {
+			const uint32_t q1 = VS<Q>::cx[i] > 0 ? r1 + 1u : r1m, q2 = VS<Q>::cx[i] > 0 ? r2p : r2 - 1u;
+			X[i + 1] = (uint32_t)*code_at(pb, q1) | ((uint32_t)*code_at(pb, q2) << 16);
+		}
 	}
 	__builtin_amdgcn_sched_barrier(0);
 	__builtin_amdgcn_s_setprio(0);
+	{ // runs -> cells: lane l's cells 2l, 2l+1 sit in lanes 2l, 2l+1 (mod 64), low halves for l < 32
+		const int ia = (int)(((2u * lane) & 63u) * 4u), ib = (int)(((2u * lane + 1u) & 63u) * 4u);
+		const uint32_t sel = lane < 32u ? 0x05040100u : 0x07060302u;
+#pragma unroll
+		for(int i = 1; i < Q; i += 2) {
+			if(VS<Q>::cx[i] == 0) continue;
+			const uint32_t ca = (uint32_t)__builtin_amdgcn_ds_bpermute(ia, (int)X[i + 1]);
+			const uint32_t cb = (uint32_t)__builtin_amdgcn_ds_bpermute(ib, (int)X[i + 1]);
+			P[i + 1] = __builtin_amdgcn_perm(cb, ca, sel); // (cb half << 16) | ca half
+		}
+	}
 	float f[Q];
 	if(liveA) {
 #pragma unroll
@@ -901,6 +920,17 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 #pragma unroll
 		for(int i = 1; i < Q; i += 2) { const uint32_t t = (P[i] ^ P[i + 1]) & 0xFFFF0000u; P[i] ^= t; P[i + 1] ^= t; }
 	}
+	{ // cells -> runs: cell c0+m sits in lane m/2, cell c0+64+m in lane 32+m/2, high halves for odd m
+		const int ja = (int)((lane >> 1) * 4u), jb = (int)((32u + (lane >> 1)) * 4u);
+		const uint32_t sel = (lane & 1u) ? 0x07060302u : 0x05040100u;
+#pragma unroll
+		for(int i = 1; i < Q; i += 2) {
+			if(VS<Q>::cx[i] == 0) continue;
+			const uint32_t ca = (uint32_t)__builtin_amdgcn_ds_bpermute(ja, (int)P[i]);
+			const uint32_t cb = (uint32_t)__builtin_amdgcn_ds_bpermute(jb, (int)P[i]);
+			X[i] = __builtin_amdgcn_perm(cb, ca, sel);
+		}
+	}
 	// store_f's order: population i to the shifted location, i+1 to the cell's own slot
 	__builtin_nontemporal_store(P[0], word_at(global_row(fi + n0), xa));
 #pragma unroll
@@ -910,8 +940,9 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 		gcode* pb = global_row(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]));
 		if(VS<Q>::cx[i] == 0) __builtin_nontemporal_store(P[i], word_at(pb, xa));
 		else {
-			*code_at(pb, s.X[VS<Q>::cx[i] + 1]) = (uint16_t)P[i];
-			*code_at(pb, XB[VS<Q>::cx[i] + 1]) = (uint16_t)(P[i] >> 16);
+			const uint32_t q1 = VS<Q>::cx[i] > 0 ? r1 + 1u : r1m, q2 = VS<Q>::cx[i] > 0 ? r2p : r2 - 1u;
+			*code_at(pb, q1) = (uint16_t)X[i];
+			*code_at(pb, q2) = (uint16_t)(X[i] >> 16);
 		}
 		__builtin_nontemporal_store(P[i + 1], word_at(global_row(fi + sa * N + n0), xa));
 	}

@@ -170,12 +170,19 @@ constexpr bool tune_offsets32(int Q, uint32_t ST, bool TRT, uint32_t EXT) {
 	if(Q == 27) return !TRT || (EXT != 0u && EXT != EXT_VOLUME_FORCE);
 	return TRT && EXT != EXT_VOLUME_FORCE;
 }
-// Two cells per lane (k_stream_collide_pair, where its measurements are): the D3Q19 SRT
-// 16-bit kernels without extensions, with EQUILIBRIUM_BOUNDARIES and with
-// EQUILIBRIUM_BOUNDARIES + FORCE_FIELD.
+// Two cells per lane (k_stream_collide_pair) for the D3Q19 16-bit kernels where it measured
+// faster than one cell per lane (256^3, same process, bit-exact; FP16S / FP16C): SRT none,
+// EQUILIBRIUM_BOUNDARIES and + FORCE_FIELD +8 to +13 %; TRT none +1.9 / +4.1 %,
+// EQUILIBRIUM_BOUNDARIES +3.6 / +0.3, + FORCE_FIELD +3.9 / +0.3, porous +1.5 / +7.3; SRT
+// VOLUME_FORCE +1.6 / -0.7 and SRT porous -0.3 / +3.7 (each for the storage it gains on).
+// TRT VOLUME_FORCE (-1.3 / +0.1), the all-extension kernels (SRT -7 to -8 %; and the pair
+// kernel has no MOVING_BOUNDARIES) and D3Q27 (133-156 VGPRs, -13 to +0.3 %) keep one cell per lane.
 constexpr bool tune_pairs(int Q, uint32_t ST, bool TRT, uint32_t EXT) {
-	return Q == 19 && ST != ST_FP32 && !TRT &&
-	       (EXT == 0u || EXT == EXT_EQUILIBRIUM_BOUNDARIES || EXT == (EXT_EQUILIBRIUM_BOUNDARIES | EXT_FORCE_FIELD));
+	constexpr uint32_t POROUS = EXT_VOLUME_FORCE | EXT_POROSITY | EXT_PRECISE_DENSITY;
+	if(Q != 19 || ST == ST_FP32 || (EXT & (EXT_MOVING_BOUNDARIES | EXT_SUBGRID)) != 0u) return false;
+	if(EXT == EXT_VOLUME_FORCE) return !TRT && ST == ST_FP16S;
+	if(EXT == POROUS) return TRT || ST == ST_FP16C;
+	return EXT == 0u || EXT == EXT_EQUILIBRIUM_BOUNDARIES || EXT == (EXT_EQUILIBRIUM_BOUNDARIES | EXT_FORCE_FIELD);
 }
 // Above 4 GiB of DDFs the FP32 kernels stream faster on the cell-linear grid than on the
 // row grid, also where the rows start on lines (same residency cap).  Plain D3Q19: 384^3

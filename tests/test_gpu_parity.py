@@ -76,8 +76,8 @@ def compare_state(lbm, ref, oracle, storage, tag):
 
 # 20x12x10, 150x6x5, 520x3x3: rows off the 128-byte lines (cell-linear grid for 16-bit, and
 # for FP32 where Nx*4 misses them); 64x5x4: rows on lines, one block; 512x4x3: one block per
-# row, and two cells per lane for the D3Q19 SRT 16-bit kernels; 1024x3x2: rows of several
-# blocks on the XCD-mapped grid, with both kernel forms
+# row, and two cells per lane for the D3Q19 16-bit kernels that take it; 1024x3x2: rows of
+# several blocks on the XCD-mapped grid, with both kernel forms
 GRIDS = {"20x12x10": (20, 12, 10), "150x6x5": (150, 6, 5), "520x3x3": (520, 3, 3), "64x5x4": (64, 5, 4), "512x4x3": (512, 4, 3),
          "1024x3x2": (1024, 3, 2)}
 
@@ -121,6 +121,42 @@ def test_stream_collide_matches_oracle(oracle, q, collision, storage, ext, grid)
             ref.update_fields()           # (src/lbm.cpp:184-191, src/lbm.hpp:390-393)
         assert np.array_equal(lbm.rho.host, ref.rho), "rho differs after update_fields"
         assert np.array_equal(lbm.u.host, ref.u), "u differs after update_fields"
+        assert abs(lbm.mass() - ref.mass()) <= 1e-12 * abs(ref.mass())
+    finally:
+        lbm.close()
+
+
+PAIR_SETS = {"equilibrium": ("EQUILIBRIUM_BOUNDARIES",), "porous": ("VOLUME_FORCE", "POROSITY", "PRECISE_DENSITY")}
+
+
+@pytest.mark.parametrize("collision,storage,ext", list(itertools.product(("SRT", "TRT"), ("FP16S", "FP16C"), PAIR_SETS)))
+def test_pair_kernel_dead_waves_and_lanes(oracle, collision, storage, ext):
+    """Two cells per lane (D3Q19 16-bit, rows of whole waves of 128 cells): rows whose first
+    or last wave is all solid (the wave returns before its exchanges), rows with both cells
+    of every fourth lane solid (the lane still passes codes between the x-shifted runs and
+    the cells' lanes and writes its own back unchanged), scattered single solid cells and,
+    with EQUILIBRIUM_BOUNDARIES, TYPE_E planes at the row ends, on 256x5x4 with rows of one
+    block; 6 steps, both parities, against the oracle."""
+    Nx, Ny, Nz = 256, 5, 4
+    names = PAIR_SETS[ext]
+    rho, u, flags = scenes.random_scene(Nx, Ny, Nz, seed=23, solid_fraction=0.05, equilibrium_faces="EQUILIBRIUM_BOUNDARIES" in names)
+    if "PRECISE_DENSITY" in names:
+        u *= np.float32(0.02)
+    x, y, z = scenes.coords(Nx, Ny, Nz)
+    flags[(y == 1) & (x < 128)] = scenes.TYPE_S               # a row's first wave
+    flags[(y == 2) & (z == 3) & (x >= 128)] = scenes.TYPE_S   # a row's last wave (the +x wrap's source)
+    flags[(y == 3) & (x % 8 < 2)] = scenes.TYPE_S             # lanes with both cells solid
+    N = Nx * Ny * Nz
+    alpha = np.random.default_rng(5).uniform(0, 1, N).astype(np.float32) if "POROSITY" in names else None
+    f = (2e-5, -1e-5, 3e-6) if "VOLUME_FORCE" in names else (0.0, 0.0, 0.0)
+    lbm, ref = setup_pair(oracle, Nx, Ny, Nz, 0.048, 19, collision, storage, names, rho, u, flags, None, alpha, f)
+    try:
+        lbm.initialize()
+        ref.initialize()
+        for s in range(6):
+            lbm.run(1)
+            ref.step(1)
+            compare_state(lbm, ref, oracle, storage, f"after step {s + 1}")
         assert abs(lbm.mass() - ref.mass()) <= 1e-12 * abs(ref.mass())
     finally:
         lbm.close()

@@ -86,16 +86,15 @@ int main() {
     LaunchShape p = pair_shape(d[0], d[1], d[2]);
     printf("pair %u %u %u %u %u %u %u %u %d\n", d[0], d[1], d[2], p.block.x, p.grid.x, p.grid.y, p.grid.z, p.xchunks, (int)p.pairs);
   }
-  printf("tunepairs %d %d %d %d %d %d %d %d\n", (int)tune_pairs(19, ST_FP16S, false, 0u), (int)tune_pairs(19, ST_FP16C, false, EXT_EQUILIBRIUM_BOUNDARIES),
-         (int)tune_pairs(19, ST_FP16S, false, EXT_EQUILIBRIUM_BOUNDARIES | EXT_FORCE_FIELD), (int)tune_pairs(19, ST_FP32, false, 0u),
-         (int)tune_pairs(19, ST_FP16S, true, 0u), (int)tune_pairs(27, ST_FP16S, false, 0u), (int)tune_pairs(19, ST_FP16S, false, EXT_VOLUME_FORCE),
-         (int)tune_pairs(19, ST_FP16C, false, EXT_ALL));
   {
     constexpr uint32_t E = EXT_EQUILIBRIUM_BOUNDARIES, VF = EXT_VOLUME_FORCE, FF = EXT_FORCE_FIELD, POR = EXT_VOLUME_FORCE | EXT_POROSITY | EXT_PRECISE_DENSITY;
     const uint32_t exts[6] = {0u, E, VF, E | FF, POR, EXT_ALL};
     for (int st = 0; st < 3; st++) for (int trt = 0; trt < 2; trt++) for (int q : {19, 27}) {
       printf("off32 %d %d %d", q, st, trt);
       for (uint32_t e : exts) printf(" %d", (int)tune_offsets32(q, (uint32_t)st, trt != 0, e));
+      printf("\n");
+      printf("tunepairs %d %d %d", q, st, trt);
+      for (uint32_t e : exts) printf(" %d", (int)tune_pairs(q, (uint32_t)st, trt != 0, e));
       printf("\n");
     }
   }
@@ -253,8 +252,9 @@ def test_pair_grid_covers_every_lane(launch_facts):
     """k_stream_collide_pair: Nx/2 lanes per row (two cells each), blocks of 64/128/256
     lanes, every chunk of every row once (XCD-mapped when a row takes several blocks, the
     3-D row grid above the launch limit), within 2^32-1 work-items; selected only for the
-    D3Q19 SRT 16-bit kernels without extensions, with EQUILIBRIUM_BOUNDARIES and with
-    EQUILIBRIUM_BOUNDARIES + FORCE_FIELD."""
+    D3Q19 16-bit kernels where it measured faster: SRT and TRT without extensions, with
+    EQUILIBRIUM_BOUNDARIES (+ FORCE_FIELD); TRT porous; SRT VOLUME_FORCE for FP16S and SRT
+    porous for FP16C; never with MOVING_BOUNDARIES (the all-extension kernels)."""
     lines = launch_facts[3]
     limit = 2 ** 32 - 1
     assert len(lines["pair"]) == 12
@@ -268,7 +268,15 @@ def test_pair_grid_covers_every_lane(launch_facts):
             assert (gy, gz) == (1, 1) and chunks == row_chunks and gx == -(-(Ny * Nz) // 8) * 8 * chunks
         else:
             assert (gx, gy, gz) == (row_chunks, Ny, Nz)
-    assert lines["tunepairs"] == [(1, 1, 1, 0, 0, 0, 0, 0)]
+    table = {(q, st, trt): flags for q, st, trt, *flags in lines["tunepairs"]}
+    assert len(table) == 12
+    for (q, st, trt), flags in table.items():   # none, E, VF, E+FF, porous, all
+        if q == 27 or st == 0:
+            assert flags == [0, 0, 0, 0, 0, 0], (q, st, trt)
+        elif trt:
+            assert flags == [1, 1, 0, 1, 1, 0], (q, st, trt)
+        else:
+            assert flags == [1, 1, int(st == 1), 1, int(st == 2), 0], (q, st, trt)
     assert dict((r[0], r[1]) for r in lines["pairrows"]) == {64: 0, 128: 1, 192: 0, 256: 1, 512: 1, 1536: 1, 4096: 1}
 
 
@@ -276,8 +284,8 @@ def test_offsets32_selection(launch_facts):
     """32-bit slot offsets on the row grid only for the 16-bit kernels where they measured
     faster: D3Q27 every SRT subset but the all-extension one, TRT with
     EQUILIBRIUM_BOUNDARIES (+ FORCE_FIELD) and the porous subset; D3Q19 the TRT subsets
-    other than VOLUME_FORCE and the all-extension one (the D3Q19 SRT kernels without
-    extensions or with EQUILIBRIUM_BOUNDARIES (+ FORCE_FIELD) take two cells per lane)."""
+    other than VOLUME_FORCE and the all-extension one, where rows of whole waves of lanes
+    do not take two cells per lane instead."""
     table = {(q, st, trt): flags for q, st, trt, *flags in launch_facts[3]["off32"]}
     assert len(table) == 12
     for (q, st, trt), flags in table.items():   # none, E, VF, E+FF, porous, all
