@@ -363,9 +363,54 @@ __device__ __forceinline__ Site make_site(uint32_t x, uint32_t y, uint32_t z, ui
 // O32: the same 32-bit offsets on the row grid (the row grid's access rule), for lattices
 // whose slots are below 4 GiB; tune_offsets32 selects the kernels that gain from it.
 template<int Q, uint32_t ST> constexpr bool slot_offsets32(bool SC) { return SC && ST != ST_FP32; }
-template<int Q, uint32_t ST, bool U = false, bool SC = false, bool O32 = false>
+// BUF: the row grid through buffer instructions, for a DDF array below 4 GiB.  The address
+// of every access is the array's base (a buffer resource in scalar registers) plus a scalar
+// offset (slot and row, wave-uniform) plus a 32-bit lane offset, of which there are three
+// per lane (x-1, x, x+1): no per-access vector address arithmetic, and no 64-bit addresses
+// held in vector registers from the loads to the stores.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ddf_buffer(const void* fi) {
+	return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(fi), (short)0, (int)0xFFFFFFFFu, 0x00020000);
+}
+// uniform_bytes is the same in every lane; readfirstlane says so to the compiler, which
+// otherwise may not see it through the parity selects and wraps the access in a loop over
+// the distinct values.  NT: non-temporal (cache policy bit 1).
+template<typename E, bool NT> __device__ __forceinline__ E buffer_elem(__amdgpu_buffer_rsrc_t r, uint32_t lane_bytes, uint32_t uniform_bytes) {
+	const uint32_t u = __builtin_amdgcn_readfirstlane(uniform_bytes);
+	if constexpr(sizeof(E) == 2) return __builtin_amdgcn_raw_buffer_load_b16(r, lane_bytes, u, NT ? 2 : 0);
+	else return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(r, lane_bytes, u, NT ? 2 : 0));
+}
+template<typename E, bool NT> __device__ __forceinline__ void buffer_elem(__amdgpu_buffer_rsrc_t r, uint32_t lane_bytes, uint32_t uniform_bytes, E v) {
+	const uint32_t u = __builtin_amdgcn_readfirstlane(uniform_bytes);
+	if constexpr(sizeof(E) == 2) __builtin_amdgcn_raw_buffer_store_b16(v, r, lane_bytes, u, NT ? 2 : 0);
+	else __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, lane_bytes, u, NT ? 2 : 0);
+}
+// the stored element of a population value
+template<uint32_t ST> __device__ __forceinline__ typename Storage<ST>::E stored(float x) {
+	if constexpr(ST == ST_FP32) return x;
+	else return Storage<ST>::encode(x);
+}
+template<int Q, uint32_t ST, bool U = false, bool SC = false, bool O32 = false, bool BUF = false>
 __device__ __forceinline__ void load_raw(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, typename Storage<ST>::E* r) {
 	typedef Storage<ST> S;
+	if constexpr(BUF) {
+		static_assert(!SC, "buffer addressing: the row grid");
+		typedef typename Storage<ST>::E E;
+		constexpr uint32_t eb = sizeof(E);
+		constexpr bool shifted_nt = ST == ST_FP32 && Q == 19; // x-shifted loads: raw_shifted's rule
+		const __amdgpu_buffer_rsrc_t rs = ddf_buffer(fi);
+		const uint32_t slot = (uint32_t)N * eb, n0 = (uint32_t)s.R[4] * eb;
+		const uint32_t v[3] = {s.X[0] * eb, s.x * eb, s.X[2] * eb};
+		r[0] = buffer_elem<E, true>(rs, v[1], n0);
+#pragma unroll
+		for(int i = 1; i < Q; i += 2) {
+			const uint32_t sa = odd ? (uint32_t)i : (uint32_t)(i + 1), sb = odd ? (uint32_t)(i + 1) : (uint32_t)i;
+			r[i] = buffer_elem<E, true>(rs, v[1], sa * slot + n0);
+			const uint32_t ob = sb * slot + (uint32_t)s.row(VS<Q>::cy[i], VS<Q>::cz[i]) * eb;
+			if(VS<Q>::cx[i] != 0) r[i + 1] = buffer_elem<E, shifted_nt>(rs, v[VS<Q>::cx[i] + 1], ob);
+			else r[i + 1] = buffer_elem<E, true>(rs, v[1], ob);
+		}
+		return;
+	}
 	if constexpr(O32 || slot_offsets32<Q, ST>(SC)) {
 		const uint32_t c = (uint32_t)s.R[4] + s.x; // the cell's index
 		r[0] = S::raw(fi, c);
@@ -393,16 +438,33 @@ __device__ __forceinline__ void load_raw(const typename Storage<ST>::E* fi, uint
 		else r[i + 1] = VS<Q>::cx[i] != 0 ? S::template raw_shifted<Q>(pb, xb) : S::raw(pb, xb);
 	}
 }
-template<int Q, uint32_t ST, bool U = false, bool SC = false, bool O32 = false>
+template<int Q, uint32_t ST, bool U = false, bool SC = false, bool O32 = false, bool BUF = false>
 __device__ __forceinline__ void load_f(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, float* f) {
 	typename Storage<ST>::E r[Q];
-	load_raw<Q, ST, U, SC, O32>(fi, N, s, odd, r);
+	load_raw<Q, ST, U, SC, O32, BUF>(fi, N, s, odd, r);
 #pragma unroll
 	for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
 }
-template<int Q, uint32_t ST, bool U = false, bool SC = false, bool O32 = false>
+template<int Q, uint32_t ST, bool U = false, bool SC = false, bool O32 = false, bool BUF = false>
 __device__ __forceinline__ void store_f(typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, const float* f) {
 	typedef Storage<ST> S;
+	if constexpr(BUF) {
+		typedef typename Storage<ST>::E E;
+		constexpr uint32_t eb = sizeof(E);
+		const __amdgpu_buffer_rsrc_t rs = ddf_buffer(fi);
+		const uint32_t slot = (uint32_t)N * eb, n0 = (uint32_t)s.R[4] * eb;
+		const uint32_t v[3] = {s.X[0] * eb, s.x * eb, s.X[2] * eb};
+		buffer_elem<E, true>(rs, v[1], n0, stored<ST>(f[0]));
+#pragma unroll
+		for(int i = 1; i < Q; i += 2) {
+			const uint32_t sa = odd ? (uint32_t)i : (uint32_t)(i + 1), sb = odd ? (uint32_t)(i + 1) : (uint32_t)i;
+			const uint32_t ob = sb * slot + (uint32_t)s.row(VS<Q>::cy[i], VS<Q>::cz[i]) * eb;
+			if(VS<Q>::cx[i] != 0) buffer_elem<E, false>(rs, v[VS<Q>::cx[i] + 1], ob, stored<ST>(f[i]));
+			else buffer_elem<E, true>(rs, v[1], ob, stored<ST>(f[i]));
+			buffer_elem<E, true>(rs, v[1], sa * slot + n0, stored<ST>(f[i + 1]));
+		}
+		return;
+	}
 	if constexpr(O32 || slot_offsets32<Q, ST>(SC)) {
 		const uint32_t c = (uint32_t)s.R[4] + s.x;
 		S::store(fi, c, f[0]);
@@ -769,7 +831,7 @@ template<bool XCD, bool LIN> __device__ __forceinline__ bool cell_coords(const L
 	}
 }
 // One lane per cell.
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD, bool XCD, bool LIN, bool O32 = false>
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool RECORD, bool XCD, bool LIN, bool O32 = false, bool BUF = false>
 __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 	a.ext &= EXT;
 	if constexpr(constant_vf_mask<TRT>(Q, ST, EXT, LIN)) a.ext = EXT_VOLUME_FORCE;
@@ -789,24 +851,24 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 		// convert each 16-bit value right after its load and wait for it, one at a time
 		E r[Q];
 		if constexpr(load_priority<TRT>(EXT)) __builtin_amdgcn_s_setprio(1);
-		load_raw<Q, ST, U, LIN, O32>(fi, a.N, s, a.odd, r);
+		load_raw<Q, ST, U, LIN, O32, BUF>(fi, a.N, s, a.odd, r);
 		__builtin_amdgcn_sched_barrier(0);
 		if constexpr(load_priority<TRT>(EXT)) __builtin_amdgcn_s_setprio(0);
 #pragma unroll
 		for(int i = 0; i < Q; i++) f[i] = Storage<ST>::decode(r[i]);
 	} else {
-		load_f<Q, ST, U, LIN, O32>(fi, a.N, s, a.odd, f);
+		load_f<Q, ST, U, LIN, O32, BUF>(fi, a.N, s, a.odd, f);
 	}
 	if((a.ext & EXT_MOVING_BOUNDARIES) && bo == TYPE_MS) moving_boundaries<Q>(a, s, f);
 	collide_cell<Q, TRT, EXT, RECORD>(a, n, bo, f, moments);
-	store_f<Q, ST, U, LIN, O32>(fi, a.N, s, a.odd, f);
+	store_f<Q, ST, U, LIN, O32, BUF>(fi, a.N, s, a.odd, f);
 }
 // XCD: the kernel may run on the 1-D XCD-mapped grid (a.xchunks > 0); false compiles the
 // mapping out.  LIN: the cell-linear grid (cell_coords), for lattices whose rows do not
 // start on 128-byte lines.
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false, bool LIN = false, bool O32 = false>
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool XCD = false, bool LIN = false, bool O32 = false, bool BUF = false>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide(LbmArgs a) {
-	stream_collide_cell<Q, ST, TRT, EXT, false, XCD, LIN, O32>(a, nullptr);
+	stream_collide_cell<Q, ST, TRT, EXT, false, XCD, LIN, O32, BUF>(a, nullptr);
 }
 // stream_collide with two adjacent cells per lane, for the 16-bit formats on rows of whole
 // waves of lanes (Nx a multiple of 128, pair_rows).  A wave takes the 128 cells c0 .. c0+127

@@ -12,6 +12,7 @@ struct LaunchShape {
 	bool lin = false;     // cell-linear grid (cell_coords in lbm_device.hpp), see linear_shape
 	bool pairs = false;   // two cells per lane (k_stream_collide_pair), see pair_shape
 	bool off32 = false;   // row grid with 32-bit slot offsets (O32 of load_raw/store_f)
+	bool buf = false;     // row grid through buffer instructions (BUF of load_raw/store_f)
 };
 // A launch holds at most 2^32-1 work-items along each grid dimension (grid size times
 // block size); lattices of more than ~4.29 G cells (a D3Q19 FP16 lattice of up to ~5.2 G
@@ -122,6 +123,7 @@ struct LbmKernels {
 	bool stream_collide_linear_xcd;   // XCD-aware segment order on the cell-linear grid
 	bool stream_collide_pairs;        // k_stream_collide_pair on rows that start on lines
 	bool stream_collide_offsets32;    // 32-bit slot offsets on the row grid (slots below 4 GiB)
+	bool stream_collide_buffer;       // buffer instructions on the row grid (DDFs below 4 GiB)
 };
 
 constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY |
@@ -184,6 +186,26 @@ constexpr bool tune_pairs(int Q, uint32_t ST, bool TRT, uint32_t EXT) {
 	if(EXT == POROUS) return TRT || ST == ST_FP16C;
 	return EXT == 0u || EXT == EXT_EQUILIBRIUM_BOUNDARIES || EXT == (EXT_EQUILIBRIUM_BOUNDARIES | EXT_FORCE_FIELD);
 }
+// Buffer instructions on the row grid (load_raw's BUF: one buffer resource, scalar slot and
+// row offsets, three lane offsets) for the 16-bit kernels that keep one cell per lane: their
+// vector registers fall from 59-100 to 43-85 (the C3-type D3Q27 TRT FP16C kernel with
+// EQUILIBRIUM_BOUNDARIES + FORCE_FIELD 96 -> 48) with no 64-bit addresses held from the loads
+// to the stores.  Measured at 256^3 against the kernel selected before (32-bit offsets where
+// tune_offsets32 says so), same process, bit-exact (FP16S / FP16C): D3Q27 SRT
+// EQUILIBRIUM_BOUNDARIES +0.9 / +3.0 %, + FORCE_FIELD +1.0 / +2.2, porous +11.4 / +15.1,
+// all extensions +22.7 / +20.6, VOLUME_FORCE -1.1 / +4.8; D3Q27 TRT none +1.2 / +2.3,
+// EQUILIBRIUM_BOUNDARIES +0.4 / +3.4, VOLUME_FORCE +3.6 / +3.4, + FORCE_FIELD +0.8 / +2.4
+// (+3.4 % for FP16C at 512x256x256), porous +10.1 / +3.7, all +15.6 / +16.9; D3Q19 TRT
+// VOLUME_FORCE +6.2 / +3.9, TRT all +2.8 / +4.6, SRT porous (FP16S) +1.1, SRT VOLUME_FORCE
+// (FP16C) +2.4.  Not taken: D3Q27 SRT none (-3.3 / -3.7), D3Q27 SRT VOLUME_FORCE FP16S,
+// D3Q19 SRT all extensions (-2.2 / -1.5), every FP32 kernel (-9 to +1 %: they wait on HBM),
+// and the D3Q19 kernels of tune_pairs (two cells per lane; on rows of half a wave they keep
+// the one-cell form they had).
+constexpr bool tune_buffer(int Q, uint32_t ST, bool TRT, uint32_t EXT) {
+	if(ST == ST_FP32) return false;
+	if(Q == 27) return TRT || (EXT != 0u && !(EXT == EXT_VOLUME_FORCE && ST == ST_FP16S));
+	return !tune_pairs(Q, ST, TRT, EXT) && (TRT || EXT != EXT_ALL);
+}
 // Above 4 GiB of DDFs the FP32 kernels stream faster on the cell-linear grid than on the
 // row grid, also where the rows start on lines (same residency cap).  Plain D3Q19: 384^3
 // +10 %, 256x512x512 +2 %, 512^3 +12 to +16 %, 768x512x512 +2.5 %, 1024x512x512 +11 %;
@@ -215,6 +237,9 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 		b.xchunks = s.xchunks;
 		if constexpr(pairs) {
 			if(s.pairs) { k_stream_collide_pair<Q, ST, TRT, EXT><<<s.grid, s.block, s.lds, st>>>(b); return; }
+		}
+		if constexpr(buffer) {
+			if(s.buf) { k_stream_collide<Q, ST, TRT, EXT, xcd_map, false, false, true><<<s.grid, s.block, s.lds, st>>>(b); return; }
 		}
 		if constexpr(offsets32) {
 			if(s.off32) { k_stream_collide<Q, ST, TRT, EXT, xcd_map, false, true><<<s.grid, s.block, s.lds, st>>>(b); return; }
@@ -250,6 +275,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static constexpr bool linear_xcd = tune_linear_xcd(Q, ST, EXT);
 	static constexpr bool pairs = tune_pairs(Q, ST, TRT, EXT);
 	static constexpr bool offsets32 = tune_offsets32(Q, ST, TRT, EXT);
+	static constexpr bool buffer = tune_buffer(Q, ST, TRT, EXT);
 	template<int AXIS> static void transfer_axis(void* fi, void* bp, void* bm, uint32_t Nx, uint32_t Ny, uint32_t Nz, bool insert, uint32_t odd, hipStream_t st) {
 		typedef typename Storage<ST>::E E;
 		const uint64_t A = AXIS == 0 ? (uint64_t)Ny * Nz : AXIS == 1 ? (uint64_t)Nx * Nz : (uint64_t)Nx * Ny;
@@ -265,7 +291,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static const LbmKernels* table() {
 		static const LbmKernels k = {initialize, stream_collide, update_fields, update_force_field, mass_partial, waves_cap,
 		                             stream_collide_record, adjoint_step, objective_partial, xcd_map, transfer_fi,
-		                             update_moving_boundaries, tune_linear_large(Q, ST, EXT), linear_xcd, pairs, offsets32};
+		                             update_moving_boundaries, tune_linear_large(Q, ST, EXT), linear_xcd, pairs, offsets32, buffer};
 		return &k;
 	}
 };

@@ -96,6 +96,9 @@ int main() {
       printf("tunepairs %d %d %d", q, st, trt);
       for (uint32_t e : exts) printf(" %d", (int)tune_pairs(q, (uint32_t)st, trt != 0, e));
       printf("\n");
+      printf("tunebuf %d %d %d", q, st, trt);
+      for (uint32_t e : exts) printf(" %d", (int)tune_buffer(q, (uint32_t)st, trt != 0, e));
+      printf("\n");
     }
   }
   printf("fits %d %d %d\n", (int)grid_fits(16777215ull, 256u), (int)grid_fits(16777216ull, 256u), (int)grid_fits(4294967295ull, 1u));
@@ -138,7 +141,7 @@ def launch_facts(tmp_path_factory):
             lines.setdefault("trt", []).append((int(p[1], 16), int(p[2], 16)))
         elif p[0] == "segment":
             lines[("segment", int(p[1]))] = tuple(int(v) for v in p[2:])
-        elif p[0] in ("big", "biglin", "pair", "tunepairs", "pairrows", "off32"):
+        elif p[0] in ("big", "biglin", "pair", "tunepairs", "pairrows", "off32", "tunebuf"):
             lines.setdefault(p[0], []).append(tuple(int(v) for v in p[1:]))
         elif p[0] == "fits":
             lines["fits"] = tuple(int(v) for v in p[1:])
@@ -191,7 +194,7 @@ def test_grid_follows_row_alignment(launch_facts):
     blocks of 256 consecutive cells cover the lattice exactly once."""
     _, _, _, lines = launch_facts
     for key, v in lines.items():
-        if key in ("trt", "big", "biglin", "fits", "pair", "tunepairs", "pairrows", "off32"):
+        if key in ("trt", "big", "biglin", "fits", "pair", "tunepairs", "pairrows", "off32", "tunebuf"):
             continue
         if isinstance(key, tuple) and key[0] == "linear":
             n, xcd = key[1], key[2]
@@ -299,3 +302,23 @@ def test_offsets32_selection(launch_facts):
             assert flags == [0, 0, 0, 0, 0, 0], (q, st, trt)
         else:
             assert flags == [1, 1, 0, 1, 1, 0], (q, st, trt)
+
+
+def test_buffer_selection(launch_facts):
+    """Buffer instructions on the row grid only for the 16-bit kernels that keep one cell per
+    lane and measured faster with them: D3Q27 every subset but SRT none and SRT VOLUME_FORCE
+    with FP16S; D3Q19 the TRT VOLUME_FORCE and all-extension kernels, SRT porous (FP16S) and
+    SRT VOLUME_FORCE (FP16C); never FP32 and never a kernel that takes two cells per lane."""
+    lines = launch_facts[3]
+    buf = {(q, st, trt): flags for q, st, trt, *flags in lines["tunebuf"]}
+    pairs = {(q, st, trt): flags for q, st, trt, *flags in lines["tunepairs"]}
+    assert len(buf) == 12
+    for (q, st, trt), flags in buf.items():   # none, E, VF, E+FF, porous, all
+        if st == 0:
+            assert flags == [0, 0, 0, 0, 0, 0], (q, st, trt)
+        elif q == 27:
+            assert flags == ([1] * 6 if trt else [0, 1, int(st == 2), 1, 1, 1]), (q, st, trt)
+        else:
+            assert not any(b and p for b, p in zip(flags, pairs[(q, st, trt)])), (q, st, trt)
+            expect = [0, 0, 1, 0, 0, 1] if trt else [0, 0, int(st == 2), 0, int(st == 1), 0]
+            assert flags == expect, (q, st, trt)
