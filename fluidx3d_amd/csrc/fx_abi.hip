@@ -456,6 +456,7 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 		ctx->shape_adj = fx::linear_shape(N, false);
 	} else if(ctx->k->stream_collide_pairs && fx::pair_rows(cfg->Nx)) {
 		ctx->shape_sc = fx::pair_shape(cfg->Nx, cfg->Ny, cfg->Nz);
+		ctx->shape_sc.buf = ctx->k->stream_collide_pair_buffer && N * cfg->velocity_set * eb <= 0xFFFFFFFFull;
 	} else if(ctx->k->stream_collide_buffer && N * cfg->velocity_set * eb <= 0xFFFFFFFFull) {
 		ctx->shape_sc.buf = true; // the whole DDF array below 4 GiB: 32-bit buffer offsets
 	} else if(ctx->k->stream_collide_offsets32 && N * eb <= 0xFFFFFFFFull) {

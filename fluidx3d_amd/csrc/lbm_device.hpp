@@ -908,7 +908,9 @@ __device__ __forceinline__ gcode* global_row(const uint16_t* p) {
 }
 __device__ __forceinline__ gcode* code_at(gcode* p, uint32_t x) { return (gcode*)((__attribute__((address_space(1))) char*)p + x * 2u); }
 __device__ __forceinline__ gword* word_at(gcode* p, uint32_t x) { return (gword*)code_at(p, x); } // x even
-template<int Q, uint32_t ST, bool TRT, uint32_t EXT>
+// BUF: every access through buffer instructions (load_raw's BUF; a DDF array below 4 GiB):
+// no 64-bit store addresses rebuilt in vector registers after the collision.
+template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool BUF = false>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 	a.ext &= EXT;
 	typedef Storage<ST> S;
@@ -931,20 +933,37 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 	const uint32_t r1m = r1 == 0u ? a.Nx - 1u : r1 - 1u, r2p = r2 + 1u == a.Nx ? 0u : r2 + 1u;
 	uint16_t* fi = (uint16_t*)a.fi;
 	const uint64_t N = a.N, n0 = s.R[4];
+	// a slot row's first element (uniform): its word at xa, its codes at q
+	const auto word_load = [&](uint64_t row) -> uint32_t {
+		if constexpr(BUF) return buffer_elem<uint32_t, true>(ddf_buffer(fi), xa * 2u, (uint32_t)row * 2u);
+		else return __builtin_nontemporal_load(word_at(global_row(fi + row), xa));
+	};
+	const auto word_store = [&](uint64_t row, uint32_t w) {
+		if constexpr(BUF) buffer_elem<uint32_t, true>(ddf_buffer(fi), xa * 2u, (uint32_t)row * 2u, w);
+		else __builtin_nontemporal_store(w, word_at(global_row(fi + row), xa));
+	};
+	const auto code_load = [&](uint64_t row, uint32_t q) -> uint32_t {
+		if constexpr(BUF) return buffer_elem<uint16_t, false>(ddf_buffer(fi), q * 2u, (uint32_t)row * 2u);
+		else return *code_at(global_row(fi + row), q);
+	};
+	const auto code_store = [&](uint64_t row, uint32_t q, uint16_t h) {
+		if constexpr(BUF) buffer_elem<uint16_t, false>(ddf_buffer(fi), q * 2u, (uint32_t)row * 2u, h);
+		else *code_at(global_row(fi + row), q) = h;
+	};
 	uint32_t P[Q]; // population i of both cells, as loaded (load_raw's order): low half cell xa
 	uint32_t X[Q]; // x-shifted slots: the codes of the runs, low half cell c0+m's, high c0+64+m's
 	__builtin_amdgcn_s_setprio(1);
-	P[0] = __builtin_nontemporal_load(word_at(global_row(fi + n0), xa));
+	P[0] = word_load(n0);
 #pragma unroll
 	for(int i = 1; i < Q; i += 2) {
 		const uint64_t sa = a.odd ? (uint64_t)i : (uint64_t)(i + 1);
 		const uint64_t sb = a.odd ? (uint64_t)(i + 1) : (uint64_t)i;
-		P[i] = __builtin_nontemporal_load(word_at(global_row(fi + sa * N + n0), xa));
-		gcode* pb = global_row(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]));
-		if(VS<Q>::cx[i] == 0) P[i + 1] = __builtin_nontemporal_load(word_at(pb, xa));
+		P[i] = word_load(sa * N + n0);
+		const uint64_t rb = sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]);
+		if(VS<Q>::cx[i] == 0) P[i + 1] = word_load(rb);
 		else {
 			const uint32_t q1 = VS<Q>::cx[i] > 0 ? r1 + 1u : r1m, q2 = VS<Q>::cx[i] > 0 ? r2p : r2 - 1u;
-			X[i + 1] = (uint32_t)*code_at(pb, q1) | ((uint32_t)*code_at(pb, q2) << 16);
+			X[i + 1] = code_load(rb, q1) | (code_load(rb, q2) << 16);
 		}
 	}
 	__builtin_amdgcn_sched_barrier(0);
@@ -994,19 +1013,19 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 		}
 	}
 	// store_f's order: population i to the shifted location, i+1 to the cell's own slot
-	__builtin_nontemporal_store(P[0], word_at(global_row(fi + n0), xa));
+	word_store(n0, P[0]);
 #pragma unroll
 	for(int i = 1; i < Q; i += 2) {
 		const uint64_t sa = a.odd ? (uint64_t)i : (uint64_t)(i + 1);
 		const uint64_t sb = a.odd ? (uint64_t)(i + 1) : (uint64_t)i;
-		gcode* pb = global_row(fi + sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]));
-		if(VS<Q>::cx[i] == 0) __builtin_nontemporal_store(P[i], word_at(pb, xa));
+		const uint64_t rb = sb * N + s.row(VS<Q>::cy[i], VS<Q>::cz[i]);
+		if(VS<Q>::cx[i] == 0) word_store(rb, P[i]);
 		else {
 			const uint32_t q1 = VS<Q>::cx[i] > 0 ? r1 + 1u : r1m, q2 = VS<Q>::cx[i] > 0 ? r2p : r2 - 1u;
-			*code_at(pb, q1) = (uint16_t)X[i];
-			*code_at(pb, q2) = (uint16_t)(X[i] >> 16);
+			code_store(rb, q1, (uint16_t)X[i]);
+			code_store(rb, q2, (uint16_t)(X[i] >> 16));
 		}
-		__builtin_nontemporal_store(P[i + 1], word_at(global_row(fi + sa * N + n0), xa));
+		word_store(sa * N + n0, P[i + 1]);
 	}
 }
 

@@ -124,6 +124,7 @@ struct LbmKernels {
 	bool stream_collide_pairs;        // k_stream_collide_pair on rows that start on lines
 	bool stream_collide_offsets32;    // 32-bit slot offsets on the row grid (slots below 4 GiB)
 	bool stream_collide_buffer;       // buffer instructions on the row grid (DDFs below 4 GiB)
+	bool stream_collide_pair_buffer;  // the same for k_stream_collide_pair
 };
 
 constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY |
@@ -206,6 +207,17 @@ constexpr bool tune_buffer(int Q, uint32_t ST, bool TRT, uint32_t EXT) {
 	if(Q == 27) return TRT || (EXT != 0u && !(EXT == EXT_VOLUME_FORCE && ST == ST_FP16S));
 	return !tune_pairs(Q, ST, TRT, EXT) && (TRT || EXT != EXT_ALL);
 }
+// The two-cells-per-lane kernel through buffer instructions (its BUF): its stores no longer
+// rebuild 64-bit addresses after the collision.  Measured against global addressing, same
+// process, bit-exact (256^3, then 1024x128x96; FP16S / FP16C): SRT none -1.4 / +0.5 and
+// -0.6 / +3.4 %, EQUILIBRIUM_BOUNDARIES (+ FORCE_FIELD) -1.0 / +2.4 and +0.8 / +3.5, SRT
+// VOLUME_FORCE (FP16S) +5.6 and +4.1, SRT porous (FP16C) +1.0 and +2.7, TRT none +2.7 / +5.9
+// and +4.6 / +6.3, TRT EQUILIBRIUM_BOUNDARIES (+ FORCE_FIELD) +3.9 to +9.3, TRT porous +2.6 to
+// +3.0.  The FP16S SRT kernels without forcing keep global addressing.
+constexpr bool tune_pair_buffer(int Q, uint32_t ST, bool TRT, uint32_t EXT) {
+	const bool unforced = EXT == 0u || EXT == EXT_EQUILIBRIUM_BOUNDARIES || EXT == (EXT_EQUILIBRIUM_BOUNDARIES | EXT_FORCE_FIELD);
+	return tune_pairs(Q, ST, TRT, EXT) && !(ST == ST_FP16S && !TRT && unforced);
+}
 // Above 4 GiB of DDFs the FP32 kernels stream faster on the cell-linear grid than on the
 // row grid, also where the rows start on lines (same residency cap).  Plain D3Q19: 384^3
 // +10 %, 256x512x512 +2 %, 512^3 +12 to +16 %, 768x512x512 +2.5 %, 1024x512x512 +11 %;
@@ -236,7 +248,13 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 		LbmArgs b = a;
 		b.xchunks = s.xchunks;
 		if constexpr(pairs) {
-			if(s.pairs) { k_stream_collide_pair<Q, ST, TRT, EXT><<<s.grid, s.block, s.lds, st>>>(b); return; }
+			if(s.pairs) {
+				if constexpr(pair_buffer) {
+					if(s.buf) { k_stream_collide_pair<Q, ST, TRT, EXT, true><<<s.grid, s.block, s.lds, st>>>(b); return; }
+				}
+				k_stream_collide_pair<Q, ST, TRT, EXT><<<s.grid, s.block, s.lds, st>>>(b);
+				return;
+			}
 		}
 		if constexpr(buffer) {
 			if(s.buf) { k_stream_collide<Q, ST, TRT, EXT, xcd_map, false, false, true><<<s.grid, s.block, s.lds, st>>>(b); return; }
@@ -276,6 +294,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static constexpr bool pairs = tune_pairs(Q, ST, TRT, EXT);
 	static constexpr bool offsets32 = tune_offsets32(Q, ST, TRT, EXT);
 	static constexpr bool buffer = tune_buffer(Q, ST, TRT, EXT);
+	static constexpr bool pair_buffer = tune_pair_buffer(Q, ST, TRT, EXT);
 	template<int AXIS> static void transfer_axis(void* fi, void* bp, void* bm, uint32_t Nx, uint32_t Ny, uint32_t Nz, bool insert, uint32_t odd, hipStream_t st) {
 		typedef typename Storage<ST>::E E;
 		const uint64_t A = AXIS == 0 ? (uint64_t)Ny * Nz : AXIS == 1 ? (uint64_t)Nx * Nz : (uint64_t)Nx * Ny;
@@ -291,7 +310,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static const LbmKernels* table() {
 		static const LbmKernels k = {initialize, stream_collide, update_fields, update_force_field, mass_partial, waves_cap,
 		                             stream_collide_record, adjoint_step, objective_partial, xcd_map, transfer_fi,
-		                             update_moving_boundaries, tune_linear_large(Q, ST, EXT), linear_xcd, pairs, offsets32, buffer};
+		                             update_moving_boundaries, tune_linear_large(Q, ST, EXT), linear_xcd, pairs, offsets32, buffer, pair_buffer};
 		return &k;
 	}
 };

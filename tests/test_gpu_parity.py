@@ -257,30 +257,34 @@ def test_full_size_256_two_steps(oracle):
         lbm.close()
 
 
-@pytest.mark.parametrize("storage,reps", [("FP32", (4, 8, 8)), ("FP16S", (8, 8, 8))])
-def test_lattice_beyond_4_gib_matches_its_tile(storage, reps):
+@pytest.mark.parametrize("storage,reps,q,collision,ext", [("FP32", (4, 8, 8), 19, "SRT", ()), ("FP16S", (8, 8, 8), 19, "SRT", ()),
+                                                      ("FP16C", (6, 8, 8), 27, "TRT", ("EQUILIBRIUM_BOUNDARIES", "FORCE_FIELD"))])
+def test_lattice_beyond_4_gib_matches_its_tile(storage, reps, q, collision, ext):
     """64-bit addressing at full scale: a 512x512x256 FP32 / 512^3 FP16S D3Q19 SRT lattice
     (5.1 GB of DDFs, slot offsets up to 4.8 GB; FP32 above 4 GiB on the cell-linear grid,
     FP16S on the XCD-mapped row grid, the 64^3 lattices on the row grid) filled with a
     Taylor-Green vortex of period 64 is the 64^3 vortex repeated, so after 3 steps every
-    64^3 tile of slots 0, 9, 17 and 18 equals the 64^3 lattice's DDFs bit for bit (the 64^3
-    lattice is oracle-checked above)."""
+    64^3 tile of slots 0, 9, 17 and 18 (and 26 for D3Q27) equals the 64^3 lattice's DDFs bit
+    for bit (the 64^3 lattice is oracle-checked above).  The D3Q27 TRT FP16C kernel with
+    EQUILIBRIUM_BOUNDARIES + FORCE_FIELD (no TYPE_E cells, zero force field) takes buffer
+    instructions at 64^3 and, with 5.4 GB of DDFs at 512x512x384, 32-bit slot offsets."""
     import fluidx3d_amd as fx
     n0 = 64
     rho0, u0, _ = scenes.taylor_green(n0, n0, n0)
-    small = fx.LBM(n0, n0, n0, 1.0 / 6.0, storage=storage)
-    big = fx.LBM(n0 * reps[2], n0 * reps[1], n0 * reps[0], 1.0 / 6.0, storage=storage)
+    opts = dict(storage=storage, velocity_set=q, collision=collision, extensions=ext)
+    small = fx.LBM(n0, n0, n0, 1.0 / 6.0, **opts)
+    big = fx.LBM(n0 * reps[2], n0 * reps[1], n0 * reps[0], 1.0 / 6.0, **opts)
     try:
         small.rho.host[:] = rho0; small.u.host[:] = u0
         small.run(3)
-        ref = small.read_ddfs().reshape(19, n0, n0, n0)
+        ref = small.read_ddfs().reshape(q, n0, n0, n0)
         big.rho.host[:] = np.tile(rho0.reshape(n0, n0, n0), reps).ravel()
         for d in range(3):
             big.u.host[d * big.N:(d + 1) * big.N] = np.tile(u0[d * n0 ** 3:(d + 1) * n0 ** 3].reshape(n0, n0, n0), reps).ravel()
         big.run(3)
         esize = 4 if storage == "FP32" else 2
-        assert 19 * big.N * esize > 4 << 30
-        for i in (0, 9, 17, 18):
+        assert q * big.N * esize > 4 << 30
+        for i in (0, 9, 17, 18) + ((26,) if q == 27 else ()):
             slot = np.empty(big.N, np.float32 if storage == "FP32" else np.uint16)
             big.ctx.read(big.fi, slot, offset=i * big.N * esize)
             tiles = slot.reshape(reps[0], n0, reps[1], n0, reps[2], n0)

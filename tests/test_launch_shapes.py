@@ -99,6 +99,9 @@ int main() {
       printf("tunebuf %d %d %d", q, st, trt);
       for (uint32_t e : exts) printf(" %d", (int)tune_buffer(q, (uint32_t)st, trt != 0, e));
       printf("\n");
+      printf("tunepairbuf %d %d %d", q, st, trt);
+      for (uint32_t e : exts) printf(" %d", (int)tune_pair_buffer(q, (uint32_t)st, trt != 0, e));
+      printf("\n");
     }
   }
   printf("fits %d %d %d\n", (int)grid_fits(16777215ull, 256u), (int)grid_fits(16777216ull, 256u), (int)grid_fits(4294967295ull, 1u));
@@ -141,7 +144,7 @@ def launch_facts(tmp_path_factory):
             lines.setdefault("trt", []).append((int(p[1], 16), int(p[2], 16)))
         elif p[0] == "segment":
             lines[("segment", int(p[1]))] = tuple(int(v) for v in p[2:])
-        elif p[0] in ("big", "biglin", "pair", "tunepairs", "pairrows", "off32", "tunebuf"):
+        elif p[0] in ("big", "biglin", "pair", "tunepairs", "pairrows", "off32", "tunebuf", "tunepairbuf"):
             lines.setdefault(p[0], []).append(tuple(int(v) for v in p[1:]))
         elif p[0] == "fits":
             lines["fits"] = tuple(int(v) for v in p[1:])
@@ -194,7 +197,7 @@ def test_grid_follows_row_alignment(launch_facts):
     blocks of 256 consecutive cells cover the lattice exactly once."""
     _, _, _, lines = launch_facts
     for key, v in lines.items():
-        if key in ("trt", "big", "biglin", "fits", "pair", "tunepairs", "pairrows", "off32", "tunebuf"):
+        if key in ("trt", "big", "biglin", "fits", "pair", "tunepairs", "pairrows", "off32", "tunebuf", "tunepairbuf"):
             continue
         if isinstance(key, tuple) and key[0] == "linear":
             n, xcd = key[1], key[2]
@@ -322,3 +325,17 @@ def test_buffer_selection(launch_facts):
             assert not any(b and p for b, p in zip(flags, pairs[(q, st, trt)])), (q, st, trt)
             expect = [0, 0, 1, 0, 0, 1] if trt else [0, 0, int(st == 2), 0, int(st == 1), 0]
             assert flags == expect, (q, st, trt)
+
+
+def test_pair_buffer_selection(launch_facts):
+    """The two-cells-per-lane kernel takes buffer instructions wherever it runs, except the
+    FP16S SRT kernels without forcing (none, EQUILIBRIUM_BOUNDARIES, + FORCE_FIELD), which
+    measured 1 % faster with global addressing."""
+    lines = launch_facts[3]
+    pb = {(q, st, trt): flags for q, st, trt, *flags in lines["tunepairbuf"]}
+    pairs = {(q, st, trt): flags for q, st, trt, *flags in lines["tunepairs"]}
+    assert len(pb) == 12
+    for key, flags in pb.items():   # none, E, VF, E+FF, porous, all
+        q, st, trt = key
+        expect = [p and not (st == 1 and not trt and j in (0, 1, 3)) for j, p in enumerate(pairs[key])]
+        assert flags == [int(e) for e in expect], key
