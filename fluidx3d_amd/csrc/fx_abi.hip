@@ -453,6 +453,7 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 	const uint32_t eb = cfg->storage == FX_FP32 ? 4u : 2u;
 	if(fx::use_linear_grid(*ctx->k, cfg->Nx, eb, N, cfg->velocity_set)) {
 		ctx->shape_sc = fx::linear_shape(N, ctx->k->stream_collide_linear_xcd);
+		ctx->shape_sc.buf = ctx->k->stream_collide_linear_buffer && N * cfg->velocity_set * eb <= 0xFFFFFFFFull;
 		ctx->shape_adj = fx::linear_shape(N, false);
 	} else if(ctx->k->stream_collide_pairs && fx::pair_rows(cfg->Nx)) {
 		ctx->shape_sc = fx::pair_shape(cfg->Nx, cfg->Ny, cfg->Nz);

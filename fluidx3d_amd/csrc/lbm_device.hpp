@@ -392,8 +392,21 @@ template<uint32_t ST> __device__ __forceinline__ typename Storage<ST>::E stored(
 template<int Q, uint32_t ST, bool U = false, bool SC = false, bool O32 = false, bool BUF = false>
 __device__ __forceinline__ void load_raw(const typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, typename Storage<ST>::E* r) {
 	typedef Storage<ST> S;
-	if constexpr(BUF) {
-		static_assert(!SC, "buffer addressing: the row grid");
+	if constexpr(BUF && SC) { // the cell-linear grid: rows per lane, every shifted slot cached
+		typedef typename Storage<ST>::E E;
+		constexpr uint32_t eb = sizeof(E);
+		const __amdgpu_buffer_rsrc_t rs = ddf_buffer(fi);
+		const uint32_t slot = (uint32_t)N * eb, c = ((uint32_t)s.R[4] + s.x) * eb;
+		r[0] = buffer_elem<E, true>(rs, c, 0u);
+#pragma unroll
+		for(int i = 1; i < Q; i += 2) {
+			const uint32_t sa = odd ? (uint32_t)i : (uint32_t)(i + 1), sb = odd ? (uint32_t)(i + 1) : (uint32_t)i;
+			r[i] = buffer_elem<E, true>(rs, c, sa * slot);
+			const uint32_t o = ((uint32_t)s.row(VS<Q>::cy[i], VS<Q>::cz[i]) + s.X[VS<Q>::cx[i] + 1]) * eb;
+			r[i + 1] = buffer_elem<E, false>(rs, o, sb * slot);
+		}
+		return;
+	} else if constexpr(BUF) {
 		typedef typename Storage<ST>::E E;
 		constexpr uint32_t eb = sizeof(E);
 		constexpr bool shifted_nt = ST == ST_FP32 && Q == 19; // x-shifted loads: raw_shifted's rule
@@ -448,7 +461,21 @@ __device__ __forceinline__ void load_f(const typename Storage<ST>::E* fi, uint64
 template<int Q, uint32_t ST, bool U = false, bool SC = false, bool O32 = false, bool BUF = false>
 __device__ __forceinline__ void store_f(typename Storage<ST>::E* fi, uint64_t N, const Site& s, uint32_t odd, const float* f) {
 	typedef Storage<ST> S;
-	if constexpr(BUF) {
+	if constexpr(BUF && SC) {
+		typedef typename Storage<ST>::E E;
+		constexpr uint32_t eb = sizeof(E);
+		const __amdgpu_buffer_rsrc_t rs = ddf_buffer(fi);
+		const uint32_t slot = (uint32_t)N * eb, c = ((uint32_t)s.R[4] + s.x) * eb;
+		buffer_elem<E, true>(rs, c, 0u, stored<ST>(f[0]));
+#pragma unroll
+		for(int i = 1; i < Q; i += 2) {
+			const uint32_t sa = odd ? (uint32_t)i : (uint32_t)(i + 1), sb = odd ? (uint32_t)(i + 1) : (uint32_t)i;
+			const uint32_t o = ((uint32_t)s.row(VS<Q>::cy[i], VS<Q>::cz[i]) + s.X[VS<Q>::cx[i] + 1]) * eb;
+			buffer_elem<E, false>(rs, o, sb * slot, stored<ST>(f[i]));
+			buffer_elem<E, true>(rs, c, sa * slot, stored<ST>(f[i + 1]));
+		}
+		return;
+	} else if constexpr(BUF) {
 		typedef typename Storage<ST>::E E;
 		constexpr uint32_t eb = sizeof(E);
 		const __amdgpu_buffer_rsrc_t rs = ddf_buffer(fi);

@@ -125,6 +125,7 @@ struct LbmKernels {
 	bool stream_collide_offsets32;    // 32-bit slot offsets on the row grid (slots below 4 GiB)
 	bool stream_collide_buffer;       // buffer instructions on the row grid (DDFs below 4 GiB)
 	bool stream_collide_pair_buffer;  // the same for k_stream_collide_pair
+	bool stream_collide_linear_buffer; // the same on the cell-linear grid
 };
 
 constexpr uint32_t EXT_ALL = EXT_VOLUME_FORCE | EXT_FORCE_FIELD | EXT_EQUILIBRIUM_BOUNDARIES | EXT_UPDATE_FIELDS | EXT_POROSITY |
@@ -218,6 +219,15 @@ constexpr bool tune_pair_buffer(int Q, uint32_t ST, bool TRT, uint32_t EXT) {
 	const bool unforced = EXT == 0u || EXT == EXT_EQUILIBRIUM_BOUNDARIES || EXT == (EXT_EQUILIBRIUM_BOUNDARIES | EXT_FORCE_FIELD);
 	return tune_pairs(Q, ST, TRT, EXT) && !(ST == ST_FP16S && !TRT && unforced);
 }
+// The cell-linear grid through buffer instructions (load_raw's BUF with SC: a scalar slot
+// offset plus per-lane cell offsets).  Measured at 200^3 against that grid's 32-bit slot
+// offsets, same process, bit-exact: every 16-bit kernel measured (D3Q19 and D3Q27, SRT and
+// TRT, none / EQUILIBRIUM_BOUNDARIES / VOLUME_FORCE / + FORCE_FIELD / porous / all
+// extensions) +2 to +30 % except D3Q19 FP16C TRT VOLUME_FORCE (-0.5 %); at 300^3 plain D3Q19
+// +1.1 % (FP16S) / +3.8 % (FP16C), D3Q19 FP16S TRT EQUILIBRIUM_BOUNDARIES + FORCE_FIELD
+// +3.9 %, the C3-type D3Q27 kernel +12.9 %.  FP32 measured -4.9 to +2.2 % and keeps global
+// addressing.
+constexpr bool tune_linear_buffer(int Q, uint32_t ST, bool TRT, uint32_t EXT) { return ST != ST_FP32; }
 // Above 4 GiB of DDFs the FP32 kernels stream faster on the cell-linear grid than on the
 // row grid, also where the rows start on lines (same residency cap).  Plain D3Q19: 384^3
 // +10 %, 256x512x512 +2 %, 512^3 +12 to +16 %, 768x512x512 +2.5 %, 1024x512x512 +11 %;
@@ -244,9 +254,18 @@ inline bool use_linear_grid(const LbmKernels& k, uint32_t Nx, uint32_t elem_byte
 // extension-free kernels, EXT_ALL the ones that read the runtime extension mask.
 template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static void initialize(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_initialize<Q, ST><<<s.grid, s.block, 0, st>>>(a); }
+	// LaunchShape::buf means buffer instructions on whichever grid the shape is: the
+	// cell-linear one (lin), two cells per lane (pairs) or the row grid.
 	static void stream_collide(const LbmArgs& a, const LaunchShape& s, hipStream_t st) {
 		LbmArgs b = a;
 		b.xchunks = s.xchunks;
+		if(s.lin) {
+			if constexpr(linear_buffer) {
+				if(s.buf) { k_stream_collide<Q, ST, TRT, EXT, linear_xcd, true, false, true><<<s.grid, s.block, s.lds, st>>>(b); return; }
+			}
+			k_stream_collide<Q, ST, TRT, EXT, linear_xcd, true><<<s.grid, s.block, s.lds, st>>>(b);
+			return;
+		}
 		if constexpr(pairs) {
 			if(s.pairs) {
 				if constexpr(pair_buffer) {
@@ -262,8 +281,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 		if constexpr(offsets32) {
 			if(s.off32) { k_stream_collide<Q, ST, TRT, EXT, xcd_map, false, true><<<s.grid, s.block, s.lds, st>>>(b); return; }
 		}
-		if(s.lin) k_stream_collide<Q, ST, TRT, EXT, linear_xcd, true><<<s.grid, s.block, s.lds, st>>>(b);
-		else k_stream_collide<Q, ST, TRT, EXT, xcd_map, false><<<s.grid, s.block, s.lds, st>>>(b);
+		k_stream_collide<Q, ST, TRT, EXT, xcd_map, false><<<s.grid, s.block, s.lds, st>>>(b);
 	}
 	static void update_fields(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_fields<Q, ST, (EXT == 0u ? 0u : EXT_ALL)><<<s.grid, s.block, 0, st>>>(a); }
 	static void update_force_field(const LbmArgs& a, const LaunchShape& s, hipStream_t st) { k_update_force_field<Q, ST><<<s.grid, s.block, 0, st>>>(a); }
@@ -295,6 +313,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static constexpr bool offsets32 = tune_offsets32(Q, ST, TRT, EXT);
 	static constexpr bool buffer = tune_buffer(Q, ST, TRT, EXT);
 	static constexpr bool pair_buffer = tune_pair_buffer(Q, ST, TRT, EXT);
+	static constexpr bool linear_buffer = tune_linear_buffer(Q, ST, TRT, EXT);
 	template<int AXIS> static void transfer_axis(void* fi, void* bp, void* bm, uint32_t Nx, uint32_t Ny, uint32_t Nz, bool insert, uint32_t odd, hipStream_t st) {
 		typedef typename Storage<ST>::E E;
 		const uint64_t A = AXIS == 0 ? (uint64_t)Ny * Nz : AXIS == 1 ? (uint64_t)Nx * Nz : (uint64_t)Nx * Ny;
@@ -310,7 +329,8 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static const LbmKernels* table() {
 		static const LbmKernels k = {initialize, stream_collide, update_fields, update_force_field, mass_partial, waves_cap,
 		                             stream_collide_record, adjoint_step, objective_partial, xcd_map, transfer_fi,
-		                             update_moving_boundaries, tune_linear_large(Q, ST, EXT), linear_xcd, pairs, offsets32, buffer, pair_buffer};
+		                             update_moving_boundaries, tune_linear_large(Q, ST, EXT), linear_xcd, pairs, offsets32, buffer, pair_buffer,
+	                             linear_buffer};
 		return &k;
 	}
 };

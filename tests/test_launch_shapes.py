@@ -102,6 +102,9 @@ int main() {
       printf("tunepairbuf %d %d %d", q, st, trt);
       for (uint32_t e : exts) printf(" %d", (int)tune_pair_buffer(q, (uint32_t)st, trt != 0, e));
       printf("\n");
+      printf("tunelinbuf %d %d %d", q, st, trt);
+      for (uint32_t e : exts) printf(" %d", (int)tune_linear_buffer(q, (uint32_t)st, trt != 0, e));
+      printf("\n");
     }
   }
   printf("fits %d %d %d\n", (int)grid_fits(16777215ull, 256u), (int)grid_fits(16777216ull, 256u), (int)grid_fits(4294967295ull, 1u));
@@ -144,7 +147,7 @@ def launch_facts(tmp_path_factory):
             lines.setdefault("trt", []).append((int(p[1], 16), int(p[2], 16)))
         elif p[0] == "segment":
             lines[("segment", int(p[1]))] = tuple(int(v) for v in p[2:])
-        elif p[0] in ("big", "biglin", "pair", "tunepairs", "pairrows", "off32", "tunebuf", "tunepairbuf"):
+        elif p[0] in ("big", "biglin", "pair", "tunepairs", "pairrows", "off32", "tunebuf", "tunepairbuf", "tunelinbuf"):
             lines.setdefault(p[0], []).append(tuple(int(v) for v in p[1:]))
         elif p[0] == "fits":
             lines["fits"] = tuple(int(v) for v in p[1:])
@@ -197,7 +200,7 @@ def test_grid_follows_row_alignment(launch_facts):
     blocks of 256 consecutive cells cover the lattice exactly once."""
     _, _, _, lines = launch_facts
     for key, v in lines.items():
-        if key in ("trt", "big", "biglin", "fits", "pair", "tunepairs", "pairrows", "off32", "tunebuf", "tunepairbuf"):
+        if key in ("trt", "big", "biglin", "fits", "pair", "tunepairs", "pairrows", "off32", "tunebuf", "tunepairbuf", "tunelinbuf"):
             continue
         if isinstance(key, tuple) and key[0] == "linear":
             n, xcd = key[1], key[2]
@@ -339,3 +342,11 @@ def test_pair_buffer_selection(launch_facts):
         q, st, trt = key
         expect = [p and not (st == 1 and not trt and j in (0, 1, 3)) for j, p in enumerate(pairs[key])]
         assert flags == [int(e) for e in expect], key
+
+
+def test_linear_buffer_selection(launch_facts):
+    """On the cell-linear grid every 16-bit kernel takes buffer instructions and no FP32 one."""
+    table = {(q, st, trt): flags for q, st, trt, *flags in launch_facts[3]["tunelinbuf"]}
+    assert len(table) == 12
+    for (q, st, trt), flags in table.items():
+        assert flags == [int(st != 0)] * 6, (q, st, trt)
