@@ -121,6 +121,7 @@ fx::LbmArgs make_args(const fx_ctx* ctx, const void* fi, const float* rho, const
 	a.ext = ctx->cfg.extensions;
 	a.odd = (uint32_t)(t & 1ull);
 	a.xchunks = 0u; // set by the stream_collide launcher from its shape
+	a.reverse = 0u;
 	a.w = ctx->cfg.w;
 	a.wm = fx::trt_minus_rate(a.w);
 	a.fx = fx_; a.fy = fy_; a.fz = fz_;
@@ -463,6 +464,8 @@ int fx_ctx_create(int device, const fx_lbm_config* cfg, fx_ctx** out) {
 	} else if(ctx->k->stream_collide_offsets32 && N * eb <= 0xFFFFFFFFull) {
 		ctx->shape_sc.off32 = true; // every slot below 4 GiB: 32-bit byte offsets
 	}
+	ctx->shape_sc.alternate = N * cfg->velocity_set * eb >= fx::ALTERNATE_ORDER_BYTES;
+	ctx->shape_adj.alternate = ctx->shape_sc.alternate;
 	const uint32_t cap = ctx->k->stream_collide_waves_cap;
 	hipError_t e = hipSetDevice(device);
 	if(e == hipSuccess) {

@@ -47,6 +47,7 @@ struct LbmArgs {
 	float wm;       // TRT antisymmetric rate trt_minus_rate(w), formed once by the host
 	float fx, fy, fz;
 	uint32_t xchunks; // stream_collide on a 1-D XCD-mapped grid: x chunks per row (0: 3-D grid)
+	uint32_t reverse; // stream_collide takes the blocks in reverse order (block_coords, cell_coords)
 };
 // TRT antisymmetric relaxation rate for magic parameter Lambda = 3/16 (src/kernel.cpp:
 // 1605-1633): the same IEEE float operations on the host and the device give the same
@@ -812,8 +813,13 @@ __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint3
 // of being written back partially by two.  Returns false for the padding blocks of the
 // last group of 8 rows.
 template<bool XCD> __device__ __forceinline__ bool block_coords(const LbmArgs& a, uint32_t& chunk, uint32_t& y, uint32_t& z) {
-	if(!XCD || a.xchunks == 0u) { chunk = blockIdx.x; y = blockIdx.y; z = blockIdx.z; return true; }
-	const uint32_t L = blockIdx.x, span = 8u * a.xchunks;
+	if(!XCD || a.xchunks == 0u) {
+		chunk = blockIdx.x; y = blockIdx.y; z = blockIdx.z;
+		if(a.reverse) { y = gridDim.y - 1u - y; z = gridDim.z - 1u - z; }
+		return true;
+	}
+	// the grid is whole groups of 8 rows: reversing keeps every row's chunks on one XCD
+	const uint32_t L = a.reverse ? gridDim.x - 1u - blockIdx.x : blockIdx.x, span = 8u * a.xchunks;
 	const uint32_t g = L / span, rem = L - g * span;
 	chunk = rem >> 3;
 	const uint32_t row = g * 8u + (rem & 7u);
@@ -838,7 +844,8 @@ __host__ __device__ inline uint64_t linear_segment(uint64_t L, uint64_t blocks, 
 // segment order.  Otherwise the row grid of block_coords.  False for lanes past the lattice.
 template<bool XCD, bool LIN> __device__ __forceinline__ bool cell_coords(const LbmArgs& a, uint32_t& x, uint32_t& y, uint32_t& z) {
 	if constexpr(LIN) {
-		const uint64_t c0 = linear_segment(blockIdx.x, gridDim.x, XCD) * blockDim.x; // first cell of the block
+		const uint32_t L = a.reverse ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
+		const uint64_t c0 = linear_segment(L, gridDim.x, XCD) * blockDim.x; // first cell of the block
 		if(c0 + threadIdx.x >= a.N) return false;
 		const uint64_t r0 = c0 / a.Nx; // its row y + z*Ny
 		const uint32_t z0 = (uint32_t)(r0 / a.Ny);

@@ -13,6 +13,7 @@ struct LaunchShape {
 	bool pairs = false;   // two cells per lane (k_stream_collide_pair), see pair_shape
 	bool off32 = false;   // row grid with 32-bit slot offsets (O32 of load_raw/store_f)
 	bool buf = false;     // row grid through buffer instructions (BUF of load_raw/store_f)
+	bool alternate = false; // stream_collide takes its blocks in reverse order on odd steps
 };
 // A launch holds at most 2^32-1 work-items along each grid dimension (grid size times
 // block size); lattices of more than ~4.29 G cells (a D3Q19 FP16 lattice of up to ~5.2 G
@@ -228,6 +229,16 @@ constexpr bool tune_pair_buffer(int Q, uint32_t ST, bool TRT, uint32_t EXT) {
 // +3.9 %, the C3-type D3Q27 kernel +12.9 %.  FP32 measured -4.9 to +2.2 % and keeps global
 // addressing.
 constexpr bool tune_linear_buffer(int Q, uint32_t ST, bool TRT, uint32_t EXT) { return ST != ST_FP32; }
+// Block order of stream_collide: forward on even steps, reversed on odd ones, so that the
+// rows a step wrote last are the first the next step reads (each step updates every cell
+// independently, so the order does not change a result).  Measured against the forward
+// order every step, same process, bit-exact: 512x256x256 FP32 +4.0 %, FP16S +7.2 %, FP16C
+// +6.0 %; the C3-type D3Q27 TRT FP16C kernel +1.3 % (512x256x256) and +0.8 % (256^3);
+// 300^3 (cell-linear) FP16S +2.2 %, FP32 +1.7 %; 384x192x192 FP32 +0.6 %; 512^3 FP16S
+// +2.6 %; 1024x512x512 FP16S +1.7 %, FP32 +0.4 %; 1536x768x768 FP16S +0.6 %; at 256^3 D3Q19
+// -0.2 to +0.2 %, below it 128^3 FP16S -2.4 % and 256x256x128 FP16S -0.6 %.  Taken from
+// 768 MiB of DDFs on.
+constexpr uint64_t ALTERNATE_ORDER_BYTES = 768ull << 20;
 // Above 4 GiB of DDFs the FP32 kernels stream faster on the cell-linear grid than on the
 // row grid, also where the rows start on lines (same residency cap).  Plain D3Q19: 384^3
 // +10 %, 256x512x512 +2 %, 512^3 +12 to +16 %, 768x512x512 +2.5 %, 1024x512x512 +11 %;
@@ -259,6 +270,7 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static void stream_collide(const LbmArgs& a, const LaunchShape& s, hipStream_t st) {
 		LbmArgs b = a;
 		b.xchunks = s.xchunks;
+		b.reverse = s.alternate ? a.odd : 0u;
 		if(s.lin) {
 			if constexpr(linear_buffer) {
 				if(s.buf) { k_stream_collide<Q, ST, TRT, EXT, linear_xcd, true, false, true><<<s.grid, s.block, s.lds, st>>>(b); return; }
@@ -292,12 +304,14 @@ template<int Q, uint32_t ST, bool TRT, uint32_t EXT> struct KernelSet {
 	static void stream_collide_record(const LbmArgs& a, const LaunchShape& s, hipStream_t st, float* m) {
 		LbmArgs b = a;
 		b.xchunks = s.xchunks;
+		b.reverse = s.alternate ? a.odd : 0u;
 		if(s.lin) k_stream_collide_record<Q, ST, TRT, EXT, false, true><<<s.grid, s.block, 0, st>>>(b, m);
 		else k_stream_collide_record<Q, ST, TRT, EXT, true, false><<<s.grid, s.block, 0, st>>>(b, m);
 	}
 	static void adjoint_step(const AdjointArgs& g, const LaunchShape& s, hipStream_t st) {
 		AdjointArgs h = g;
 		h.a.xchunks = s.xchunks;
+		h.a.reverse = s.alternate ? g.a.odd : 0u;
 		if(s.lin) k_adjoint_step<Q, TRT, false, true><<<s.grid, s.block, 0, st>>>(h);
 		else k_adjoint_step<Q, TRT, true, false><<<s.grid, s.block, 0, st>>>(h);
 	}
