@@ -273,9 +273,11 @@ __global__ void k_reset3(float* F, uint64_t n3) {
 	for(uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (uint64_t)gridDim.x * blockDim.x) F[i] = 0.0f;
 }
 
+// Odd elements go through encode_high, the form the two-cells-per-lane kernel packs its
+// second cell's codes with, so one round trip checks both encoders.
 template<uint32_t ST> __global__ void k_codec(const float* x, uint16_t* codes, float* decoded, uint64_t n) {
 	for(uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-		const uint16_t c = Storage<ST>::encode(x[i]);
+		const uint16_t c = (i & 1u) ? (uint16_t)(Storage<ST>::encode_high(x[i], 0u) >> 16) : Storage<ST>::encode(x[i]);
 		codes[i] = c;
 		decoded[i] = Storage<ST>::decode(c);
 	}

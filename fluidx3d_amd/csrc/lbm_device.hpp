@@ -111,6 +111,8 @@ template<> struct Storage<ST_FP16S> {
 	__device__ static __forceinline__ float decode(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h) * 3.0517578E-5f; }
 	__device__ static __forceinline__ uint16_t encode(float x) { return __builtin_bit_cast(uint16_t, (_Float16)(x * 32768.0f)); }
 	__device__ static __forceinline__ uint32_t encode_word(float x) { return encode(x); }
+	// (code << 16) | (low & 0xFFFF)
+	__device__ static __forceinline__ uint32_t encode_high(float x, uint32_t low) { return __builtin_amdgcn_perm(encode_word(x), low, 0x05040100u); }
 	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
 	__device__ static __forceinline__ float load(const E* p, uint32_t o) { return decode(raw(p, o)); }
 	__device__ static __forceinline__ void store(E* p, uint32_t o, float x) { __builtin_nontemporal_store(encode(x), elem(p, o)); }
@@ -146,13 +148,25 @@ template<> struct Storage<ST_FP16C> {
 	// where it is below 2^11), but the compiler cannot see that and would mask the denormal
 	// branch to 16 bits, one VALU operation per DDF; 16-bit stores and the two-cell packing
 	// drop the upper bits themselves.
-	__device__ static __forceinline__ uint32_t encode_word(float x) {
-		const uint32_t b = __float_as_uint(x) + 0x00000800u;
+	// magnitude: the code without its sign, below 2^15; b = bits(x) + 0x800
+	__device__ static __forceinline__ uint32_t magnitude(float x, uint32_t b) {
 		const uint32_t normal = (b >> 12) & 0x7FFFu;
 		const float ax = fabsf(x);
 		const uint32_t denormal = __float2uint_rz(__builtin_fmaf(ax, 0x1p25f, 0x1.fffffep-2f));
-		const uint32_t mag = ax < __uint_as_float(0x387FF800u) ? denormal : normal;
-		return ((b >> 16) & 0x8000u) | mag;
+		return ax < __uint_as_float(0x387FF800u) ? denormal : normal;
+	}
+	__device__ static __forceinline__ uint32_t encode_word(float x) {
+		const uint32_t b = __float_as_uint(x) + 0x00000800u;
+		return ((b >> 16) & 0x8000u) | magnitude(x, b);
+	}
+	// (code << 16) | (low & 0xFFFF), the code of the second cell of a two-cell word: the
+	// magnitude goes to the high half with one byte permute and the sign (bit 31 of b, which
+	// is bit 15 of the code shifted by 16) is inserted in place, one operation fewer than
+	// shifting the sign down and the code up.
+	__device__ static __forceinline__ uint32_t encode_high(float x, uint32_t low) {
+		const uint32_t b = __float_as_uint(x) + 0x00000800u;
+		const uint32_t t = __builtin_amdgcn_perm(magnitude(x, b), low, 0x05040100u);
+		return (t & 0x7FFFFFFFu) | (b & 0x80000000u);
 	}
 	__device__ static __forceinline__ uint16_t encode(float x) { return (uint16_t)encode_word(x); }
 	__device__ static __forceinline__ uint16_t raw(const E* p, uint32_t o) { return __builtin_nontemporal_load(elem(p, o)); }
@@ -1030,7 +1044,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 		for(int i = 0; i < Q; i++) f[i] = S::decode((uint16_t)(P[i] >> 16));
 		collide_cell<Q, TRT, EXT, false>(a, n + 1u, flB & TYPE_BO, f, nullptr);
 #pragma unroll
-		for(int i = 0; i < Q; i++) P[i] = __builtin_amdgcn_perm(S::encode_word(f[i]), P[i], 0x05040100u); // (code << 16) | (P & 0xFFFF)
+		for(int i = 0; i < Q; i++) P[i] = S::encode_high(f[i], P[i]); // (code << 16) | (P & 0xFFFF)
 	} else {
 #pragma unroll
 		for(int i = 1; i < Q; i += 2) { const uint32_t t = (P[i] ^ P[i + 1]) & 0xFFFF0000u; P[i] ^= t; P[i + 1] ^= t; }

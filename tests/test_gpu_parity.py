@@ -298,7 +298,9 @@ def test_lattice_beyond_4_gib_matches_its_tile(storage, reps, q, collision, ext)
 def test_codec_matches_oracle(oracle, storage):
     """Device encode/decode against the oracle: every 16-bit code that encodes a finite
     value round-trips, and 2M random floats across all magnitudes encode identically
-    (FP16C: also every float of the four binades where the encoding rule changes form)."""
+    (FP16C: also every float of the four binades where the encoding rule changes form).
+    The device encodes odd elements with the second-cell form of the two-cells-per-lane
+    kernel (encode_high), even ones with the plain encoder."""
     import fluidx3d_amd as fx
     from fluidx3d_amd import _abi
     cfg = fx.make_config(4, 4, 4, 1.0)
