@@ -806,12 +806,16 @@ __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint3
 			for(int i = 0; i < Q; i++) Fin[i] = Fn[i];
 		}
 		const float hwp = 0.5f * wp, hwm = 0.5f * wm;
+		// Without VOLUME_FORCE the forcing terms are zero; the reference still adds them
+		// (fhn[i] + 0.0f), which changes only the sign of an exact zero, so that addition
+		// is left out (one VALU operation per population)
+		constexpr bool forced = (EXT & (EXT_VOLUME_FORCE | EXT_POROSITY)) != 0u;
 		float out[Q];
 #pragma unroll
 		for(int i = 0; i < Q; i++) {
 			const int b = i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); // opposite slot
 			const float c = __builtin_fmaf(hwp, ((feq[i] - fhn[i]) + feq[b]) - fhn[b],
-			                               __builtin_fmaf(hwm, ((feq[i] - feq[b]) - fhn[i]) + fhn[b], fhn[i] + Fin[i]));
+			                               __builtin_fmaf(hwm, ((feq[i] - feq[b]) - fhn[i]) + fhn[b], forced ? fhn[i] + Fin[i] : fhn[i]));
 			out[i] = isE ? feq[i] : c;
 		}
 #pragma unroll
