@@ -257,9 +257,11 @@ def test_full_size_256_two_steps(oracle):
         lbm.close()
 
 
-@pytest.mark.parametrize("storage,reps,q,collision,ext", [("FP32", (4, 8, 8), 19, "SRT", ()), ("FP16S", (8, 8, 8), 19, "SRT", ()),
-                                                      ("FP16C", (6, 8, 8), 27, "TRT", ("EQUILIBRIUM_BOUNDARIES", "FORCE_FIELD"))])
-def test_lattice_beyond_4_gib_matches_its_tile(storage, reps, q, collision, ext):
+@pytest.mark.parametrize("storage,reps,q,collision,ext,n0", [("FP32", (4, 8, 8), 19, "SRT", (), 64), ("FP16S", (8, 8, 8), 19, "SRT", (), 64),
+                                                         ("FP16C", (6, 8, 8), 27, "TRT", ("EQUILIBRIUM_BOUNDARIES", "FORCE_FIELD"), 64),
+                                                         ("FP16C", (7, 8, 8), 19, "TRT", ("EQUILIBRIUM_BOUNDARIES", "FORCE_FIELD"), 64),
+                                                         ("FP16S", (23, 25, 25), 19, "SRT", ("VOLUME_FORCE",), 20)])
+def test_lattice_beyond_4_gib_matches_its_tile(storage, reps, q, collision, ext, n0):
     """64-bit addressing at full scale: a 512x512x256 FP32 / 512^3 FP16S D3Q19 SRT lattice
     (5.1 GB of DDFs, slot offsets up to 4.8 GB; FP32 above 4 GiB on the cell-linear grid,
     FP16S on the XCD-mapped row grid, the 64^3 lattices on the row grid) filled with a
@@ -267,9 +269,13 @@ def test_lattice_beyond_4_gib_matches_its_tile(storage, reps, q, collision, ext)
     64^3 tile of slots 0, 9, 17 and 18 (and 26 for D3Q27) equals the 64^3 lattice's DDFs bit
     for bit (the 64^3 lattice is oracle-checked above).  The D3Q27 TRT FP16C kernel with
     EQUILIBRIUM_BOUNDARIES + FORCE_FIELD (no TYPE_E cells, zero force field) takes buffer
-    instructions at 64^3 and, with 5.4 GB of DDFs at 512x512x384, 32-bit slot offsets."""
+    instructions at 64^3 and, with 5.4 GB of DDFs at 512x512x384, 32-bit slot offsets; the
+    D3Q19 TRT FP16C two-cells-per-lane kernel takes buffer instructions at 64^3 and global
+    addressing with 4.5 GB at 512x512x448; the 20^3 vortex (cell-linear grid, rows off the
+    lines) tiled to 500x500x460 (4.4 GB of FP16S DDFs) leaves the cell-linear grid's buffer
+    instructions for its 32-bit slot offsets.  Every large lattice here also takes its
+    blocks in reverse order on odd steps."""
     import fluidx3d_amd as fx
-    n0 = 64
     rho0, u0, _ = scenes.taylor_green(n0, n0, n0)
     opts = dict(storage=storage, velocity_set=q, collision=collision, extensions=ext)
     small = fx.LBM(n0, n0, n0, 1.0 / 6.0, **opts)
