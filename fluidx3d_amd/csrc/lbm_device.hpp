@@ -383,6 +383,10 @@ template<int Q, uint32_t ST> constexpr bool slot_offsets32(bool SC) { return SC 
 // offset (slot and row, wave-uniform) plus a 32-bit lane offset, of which there are three
 // per lane (x-1, x, x+1): no per-access vector address arithmetic, and no 64-bit addresses
 // held in vector registers from the loads to the stores.
+// The resource: base fi, stride 0 (raw buffer), the largest record count (the range check
+// never clips an offset of a DDF array below 4 GiB), and descriptor word 3 = 0x00020000,
+// data format 32 bits, which the untyped buffer_load/store_short/dword of the gfx9 family
+// need to be a valid format (the value AMD's composable_kernel uses for gfx90a/gfx94x/gfx950).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t ddf_buffer(const void* fi) {
 	return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(fi), (short)0, (int)0xFFFFFFFFu, 0x00020000);
 }
