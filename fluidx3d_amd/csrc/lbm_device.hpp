@@ -310,12 +310,56 @@ template<int Q> __device__ __forceinline__ void calc_forcing(float ux, float uy,
 	for(int k = 1; k < Q; k += 2) fo.pair(k, Fin[k], Fin[k + 1]);
 }
 
+// Quotients a/rho with one reciprocal of rho shared between them, each equal to the IEEE
+// division a/rho.  The compiler's division takes about 11 operations per quotient: a
+// reciprocal r = rcp(d) refined by one Newton step, q = n*r, two corrections q += (n-d*q)*r,
+// with v_div_scale scaling n and d beforehand when the operands could under- or overflow
+// in between, and v_div_fixup for zero, infinite and NaN operands.  Where neither can
+// apply, that arithmetic runs as it is, and r, formed once, serves every quotient:
+// 3 operations once, 5 per quotient.  The scaling applies to none of these operands when
+// d is normal with 2^-64 <= |d| <= 2^64 and a is 0 or 2^-39 <= |a| < 2^7: then 1/d, a/d and
+// the residuals are normal, and the exponents of a and d differ by less than 96.
+// Numerators that qualify: 0.5, and the momentum j of a 16-bit kernel without
+// MOVING_BOUNDARIES, whose populations are all decoded codes, i.e. multiples of 2^-39
+// (FP16S) or 2^-25 (FP16C) below 2 in magnitude, so that every partial sum is one too and
+// |j| < 2*27.  Such a sum as the density, rho = 1 + sum f, is 0, non-finite (an FP16S code
+// of inf/NaN) or normal with 2^-39 <= |rho| < 2^6, so there a normal rho (plain_sum, one
+// class test) is enough; a preset density needs the range test (plain_any).  A lane whose
+// rho fails takes the IEEE division.
+struct SharedDivisor {
+	float d, r;
+	__device__ __forceinline__ explicit SharedDivisor(float dn) : d(dn) {
+		const float r0 = __builtin_amdgcn_rcpf(dn);
+		r = __builtin_fmaf(__builtin_fmaf(-dn, r0, 1.0f), r0, r0);
+	}
+	__device__ __forceinline__ float quotient(float a) const {
+		float q = a * r;
+		q = __builtin_fmaf(__builtin_fmaf(-d, q, a), r, q);
+		return __builtin_fmaf(__builtin_fmaf(-d, q, a), r, q);
+	}
+	__device__ __forceinline__ bool plain_sum() const { return __builtin_amdgcn_classf(d, 0x108); } // -normal, +normal
+	// |d| in [2^-64, 2^64]: (bits << 1) drops the sign; exponent fields 63..191
+	__device__ __forceinline__ bool plain_any() const {
+		return (__float_as_uint(d) << 1) - (0x1F800000u << 1) <= ((0x5F800000u - 0x1F800000u) << 1);
+	}
+};
+// The kernels that may use SharedDivisor for u = j/rho (see there).
+constexpr bool shared_division(uint32_t ST, uint32_t EXT) { return ST != ST_FP32 && !(EXT & EXT_MOVING_BOUNDARIES); }
+
 // Force block of stream_collide/update_fields (src/kernel.cpp:1549-1563, 1847-1859)
 // plus the Brinkman porosity term: u = (u + f*0.5/rho)/(1+alpha/2), F = f - alpha*rho*u.
+// SD: 0.5/rho through SharedDivisor (its quotient is the IEEE one); PRESET: rhon may be
+// a preset density (EQUILIBRIUM_BOUNDARIES) rather than a sum of decoded populations.
+template<bool SD = false, bool PRESET = true>
 __device__ __forceinline__ void force_block(uint32_t ext, float rhon, float& ux, float& uy, float& uz,
                                             float& fxn, float& fyn, float& fzn, float alphan) {
 	if(ext & (EXT_VOLUME_FORCE | EXT_POROSITY)) {
-		const float rho2 = 0.5f / rhon;
+		float rho2;
+		if constexpr(SD) { // rhon is the populations' sum or, with EQUILIBRIUM_BOUNDARIES, a preset
+			const SharedDivisor dv(rhon);
+			rho2 = dv.quotient(0.5f);
+			if(!(PRESET ? dv.plain_any() : dv.plain_sum())) rho2 = 0.5f / rhon;
+		} else rho2 = 0.5f / rhon;
 		ux = __builtin_fmaf(fxn, rho2, ux);
 		uy = __builtin_fmaf(fyn, rho2, uy);
 		uz = __builtin_fmaf(fzn, rho2, uz);
@@ -738,7 +782,8 @@ __device__ __forceinline__ void collide_pairs(float w, float wm, bool vf, bool i
 		}
 	}
 }
-template<int Q, bool TRT, uint32_t EXT, bool RECORD>
+// SD: u = j/rho and 0.5/rho through SharedDivisor; only where shared_division allows it.
+template<int Q, bool TRT, uint32_t EXT, bool RECORD, bool SD = false>
 __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint32_t bo, float* fhn, float* moments) {
 	const bool isE = (a.ext & EXT_EQUILIBRIUM_BOUNDARIES) && bo == TYPE_E;
 	float rhon, uxn, uyn, uzn, rhom1;
@@ -753,7 +798,13 @@ __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint3
 		if constexpr(RECORD) { // pre-collision moments for the adjoint sweep
 			moments[n] = rhon; moments[a.N + n] = jx; moments[2ull * a.N + n] = jy; moments[3ull * a.N + n] = jz;
 		}
-		uxn = jx / rhon; uyn = jy / rhon; uzn = jz / rhon;
+		if constexpr(SD) {
+			const SharedDivisor dv(rhon);
+			uxn = dv.quotient(jx); uyn = dv.quotient(jy); uzn = dv.quotient(jz);
+			if(!dv.plain_sum()) { uxn = jx / rhon; uyn = jy / rhon; uzn = jz / rhon; }
+		} else {
+			uxn = jx / rhon; uyn = jy / rhon; uzn = jz / rhon;
+		}
 	}
 	if(isE) { // preset density/velocity of equilibrium boundary cells
 		rhon = a.rho[n]; uxn = a.u[n]; uyn = a.u[a.N + n]; uzn = a.u[2ull * a.N + n];
@@ -762,7 +813,7 @@ __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint3
 	float fxn = a.fx, fyn = a.fy, fzn = a.fz;
 	if(a.ext & EXT_FORCE_FIELD) { fxn += a.F[n]; fyn += a.F[a.N + n]; fzn += a.F[2ull * a.N + n]; }
 	const float alphan = (a.ext & EXT_POROSITY) ? a.alpha[n] : 0.0f;
-	force_block(a.ext, rhon, uxn, uyn, uzn, fxn, fyn, fzn, alphan);
+	force_block<SD, (EXT & EXT_EQUILIBRIUM_BOUNDARIES) != 0u>(a.ext, rhon, uxn, uyn, uzn, fxn, fyn, fzn, alphan);
 	const bool vf = a.ext & (EXT_VOLUME_FORCE | EXT_POROSITY);
 	if((a.ext & EXT_UPDATE_FIELDS) && !isE) {
 		a.rho[n] = rhon; a.u[n] = uxn; a.u[a.N + n] = uyn; a.u[2ull * a.N + n] = uzn;
@@ -916,7 +967,7 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 		load_f<Q, ST, U, LIN, O32, BUF>(fi, a.N, s, a.odd, f);
 	}
 	if((a.ext & EXT_MOVING_BOUNDARIES) && bo == TYPE_MS) moving_boundaries<Q>(a, s, f);
-	collide_cell<Q, TRT, EXT, RECORD>(a, n, bo, f, moments);
+	collide_cell<Q, TRT, EXT, RECORD, shared_division(ST, EXT)>(a, n, bo, f, moments);
 	store_f<Q, ST, U, LIN, O32, BUF>(fi, a.N, s, a.odd, f);
 }
 // XCD: the kernel may run on the 1-D XCD-mapped grid (a.xchunks > 0); false compiles the
@@ -968,6 +1019,7 @@ __device__ __forceinline__ gword* word_at(gcode* p, uint32_t x) { return (gword*
 // no 64-bit store addresses rebuilt in vector registers after the collision.
 template<int Q, uint32_t ST, bool TRT, uint32_t EXT, bool BUF = false>
 __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
+	static_assert(ST != ST_FP32 && !(EXT & EXT_MOVING_BOUNDARIES), "16-bit codes only; populations as decoded");
 	a.ext &= EXT;
 	typedef Storage<ST> S;
 	uint32_t chunk, y, z;
@@ -1039,7 +1091,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 	if(liveA) {
 #pragma unroll
 		for(int i = 0; i < Q; i++) f[i] = S::decode((uint16_t)P[i]);
-		collide_cell<Q, TRT, EXT, false>(a, n, flA & TYPE_BO, f, nullptr);
+		collide_cell<Q, TRT, EXT, false, shared_division(ST, EXT)>(a, n, flA & TYPE_BO, f, nullptr);
 #pragma unroll
 		for(int i = 0; i < Q; i++) P[i] = (P[i] & 0xFFFF0000u) | S::encode_word(f[i]);
 	} else {
@@ -1050,7 +1102,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 	if(liveB) {
 #pragma unroll
 		for(int i = 0; i < Q; i++) f[i] = S::decode((uint16_t)(P[i] >> 16));
-		collide_cell<Q, TRT, EXT, false>(a, n + 1u, flB & TYPE_BO, f, nullptr);
+		collide_cell<Q, TRT, EXT, false, shared_division(ST, EXT)>(a, n + 1u, flB & TYPE_BO, f, nullptr);
 #pragma unroll
 		for(int i = 0; i < Q; i++) P[i] = S::encode_high(f[i], P[i]); // (code << 16) | (P & 0xFFFF)
 	} else {
