@@ -208,6 +208,86 @@ def test_velocity_clamp_matches_oracle(oracle, q, collision, storage, ext, grid)
         lbm.close()
 
 
+def wide_codes(storage, rng, n):
+    """Random finite 16-bit codes, half of them below 2^-6 in magnitude, half of any exponent."""
+    sign = rng.integers(0, 2, n).astype(np.uint16) << 15
+    if storage == "FP16S":  # IEEE binary16 of x*2^15: exponent fields 0..30 (31 is inf/NaN)
+        e = np.where(rng.uniform(0, 1, n) < 0.5, rng.integers(0, 9, n), rng.integers(0, 31, n)).astype(np.uint16)
+        return sign | (e << 10) | rng.integers(0, 1 << 10, n).astype(np.uint16)
+    e = np.where(rng.uniform(0, 1, n) < 0.5, rng.integers(0, 9, n), rng.integers(0, 16, n)).astype(np.uint16)
+    return sign | (e << 11) | rng.integers(0, 1 << 11, n).astype(np.uint16)
+
+
+WIDE_SETS = {"plain": (), "volume_force": ("VOLUME_FORCE",), "drag": ("EQUILIBRIUM_BOUNDARIES", "FORCE_FIELD")}
+
+
+@pytest.mark.parametrize("grid", ["512x4x3", "20x12x10"])
+@pytest.mark.parametrize("q,collision,storage,ext", list(itertools.product((19, 27), ("SRT", "TRT"), ("FP16S", "FP16C"), WIDE_SETS)))
+def test_codes_of_every_magnitude(oracle, q, collision, storage, ext, grid):
+    """One step from DDFs that are random 16-bit codes of every exponent: densities
+    rho = 1 + sum f from below -1 to above 2, velocities j/rho of any size (clamped), outputs
+    beyond the formats' range.  Planted cells load populations that give rho = 0 with j != 0
+    (u = +-inf before the clamp), rho = the smallest code (2^-39 FP16S, 2^-25 FP16C) and
+    rho = -2^-8; TYPE_E cells carry preset densities 1e-30, 1e30 and 3e-20 (the 0.5/rho of
+    the force block far outside the usual range).  These take the division's general path
+    where the kernels otherwise share one reciprocal of rho between u = j/rho and 0.5/rho.
+    Every code is compared with the oracle's."""
+    Nx, Ny, Nz = GRIDS[grid]
+    N = Nx * Ny * Nz
+    names = WIDE_SETS[ext]
+    rho, u, flags = scenes.random_scene(Nx, Ny, Nz, seed=11 + q, equilibrium_faces="EQUILIBRIUM_BOUNDARIES" in names)
+    e_cells = np.flatnonzero(flags == 2)
+    if e_cells.size:
+        rho[e_cells[:3]] = np.float32([1e-30, 1e30, 3e-20])
+    rng = np.random.default_rng(q + 5 * len(names))
+    F = (1e-3 * rng.uniform(-1, 1, 3 * N)).astype(np.float32) if "FORCE_FIELD" in names else None
+    f = (1e-3, 2e-3, 5e-4) if "VOLUME_FORCE" in names else (0.0, 0.0, 0.0)
+    lbm, ref = setup_pair(oracle, Nx, Ny, Nz, 0.1, q, collision, storage, names, rho, u, flags, F, None, f)
+    try:
+        lbm.initialize()
+        ref.initialize()
+        codes = wide_codes(storage, rng, q * N)
+        # where each cell loads population i at the next step: an FP32 lattice of indices
+        probe = oracle.OracleLBM(Nx, Ny, Nz, w=lbm.w, q=q, storage="FP32")
+        probe.fi[:] = np.arange(q * N, dtype=np.float32)
+        probe.t = ref.t
+        at = probe.populations().astype(np.int64)  # (q, N)
+        minus_one = np.uint16(0xF800)  # -1 in both formats (FP16S: -2^15 in binary16; FP16C: e=15)
+        quarter = np.uint16(0x7000 if storage == "FP16S" else 0x6800)
+        smallest = np.uint16(1)
+        fluid = np.flatnonzero((flags & 3) != 1)
+        plant = {"zero": fluid[::97], "tiny": fluid[31::97], "negative": fluid[61::97]}
+        for kind, cells in plant.items():
+            for n in cells:
+                codes[at[:, n]] = 0
+                codes[at[0, n]] = minus_one
+                if kind == "zero":  # +-x, +-y, +-z pairs cancel in rho; j = (1/2, 1/2, 1/2), u = +inf
+                    for i in range(1, 7):
+                        codes[at[i, n]] = quarter if i % 2 else quarter | np.uint16(0x8000)
+                    if F is not None:  # forces that keep u = +inf + f*0.5/rho at +inf
+                        F[[n, N + n, 2 * N + n]] = np.abs(F[[n, N + n, 2 * N + n]])
+                elif kind == "tiny":
+                    codes[at[1, n]] = smallest
+                else:  # rho = -2^-8
+                    codes[at[3, n]] = np.uint16(0xD800 if storage == "FP16S" else 0xB800)
+        lbm.write_ddfs(codes)
+        ref.fi[:] = codes
+        if F is not None:
+            lbm.F.host[:] = F
+            lbm.F.write_to_device()
+            ref.F[:] = F
+        pops = oracle.decode(storage, codes)[at.ravel()].reshape(q, N).astype(np.float64)
+        dens = 1.0 + pops.sum(axis=0)
+        assert (dens[plant["zero"]] == 0).all() and (dens[plant["tiny"]] > 0).all() and (dens[plant["tiny"]] < 1e-7).all()
+        assert (dens[plant["negative"]] == -2.0 ** -8).all()
+        assert dens[fluid].min() < -1 and dens[fluid].max() > 2
+        lbm.run(1)
+        ref.step(1)
+        compare_state(lbm, ref, oracle, storage, "after one step")
+    finally:
+        lbm.close()
+
+
 @pytest.mark.parametrize("storage", ["FP32", "FP16S", "FP16C"])
 def test_tgv_64_1000_steps(oracle, storage):
     """D3Q19 SRT 64^3 Taylor-Green, nu=1/6, u0=0.05, 1000 steps.
