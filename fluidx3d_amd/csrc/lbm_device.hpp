@@ -4,7 +4,8 @@
 //
 // Arithmetic follows the reference kernels operation by operation (src/kernel.cpp
 // :1004-1101, 1454-1636, 1794-1870): explicit fmaf() where the reference writes fma(),
-// everything else uncontracted (built with -ffp-contract=off), IEEE division.  Terms
+// everything else uncontracted (built with -ffp-contract=off), IEEE division (in the
+// 16-bit kernels u = j/rho through SharedDivisor, which gives the same quotients).  Terms
 // whose lattice coefficient is 0 are left out, as the reference's compiler does under
 // -cl-no-signed-zeros (src/opencl.hpp:316); results agree with the oracle up to the
 // sign of zero.
