@@ -194,6 +194,12 @@ def sphere_leg(fx, name, dims, radius, velocity_set, collision, storage, extensi
     lbm.t = t
     out = {"workload": name}
     out.update(rate_summary(N, fx.bandwidth_bytes_per_cell(lbm.cfg), batch_ms, steps))
+    if "FORCE_FIELD" in extensions and "VOLUME_FORCE" not in extensions:
+        # the byte count above includes 12 B/cell of force field (src/lbm.cpp:60-62), but
+        # without VOLUME_FORCE stream_collide never uses F, so neither kernel reads it
+        # (src/kernel.cpp:1497-1503 adds it to a force that only VOLUME_FORCE applies)
+        out["moved_bytes_per_cell"] = out["bytes_per_cell"] - 12
+        out["moved_GBps"] = out["achieved_GBps"] * out["moved_bytes_per_cell"] / out["bytes_per_cell"]
     out["per_step_sync_mlups"] = N / sync_s * 1e-6
     out["steps_run"] = t
     lbm.close()
@@ -318,7 +324,8 @@ def main():
             r = sphere_leg(fx, name, dims, radius, q, coll, st, ext, steps, min(a.warmup, 20), min(a.sync_steps, 50),
                            a.repeats, perturb)
             spheres[key] = {k: (float(f"{v:.6g}") if isinstance(v, float) else v) for k, v in r.items()}
-            print(f"# {key}: {r['mlups']:.0f} MLUPs/s, {r['achieved_GBps']:.0f} GB/s ({100 * r['frac']:.1f}% of 8 TB/s)",
+            moved = f"; {r['moved_GBps']:.0f} GB/s moved ({r['moved_bytes_per_cell']} B/cell)" if "moved_GBps" in r else ""
+            print(f"# {key}: {r['mlups']:.0f} MLUPs/s, {r['achieved_GBps']:.0f} GB/s ({100 * r['frac']:.1f}% of 8 TB/s){moved}",
                   file=sys.stderr, flush=True)
     adjoint = None
     if not a.no_adjoint:
