@@ -227,8 +227,9 @@ def test_codes_of_every_magnitude(oracle, q, collision, storage, ext, grid):
     """One step from DDFs that are random 16-bit codes of every exponent: densities
     rho = 1 + sum f from below -1 to above 2, velocities j/rho of any size (clamped), outputs
     beyond the formats' range.  Planted cells load populations that give rho = 0 with j != 0
-    (u = +-inf before the clamp), rho = the smallest code (2^-39 FP16S, 2^-25 FP16C) and
-    rho = -2^-8; TYPE_E cells carry preset densities 1e-30, 1e30 and 3e-20 (the 0.5/rho of
+    (u = +-inf before the clamp), rho = the smallest code (2^-39 FP16S, 2^-25 FP16C),
+    rho = -2^-8 and rho = 2 - 2^-23 (an all-ones mantissa, the divisor for which a
+    reciprocal one ulp off would misround the quotient); TYPE_E cells carry preset densities 1e-30, 1e30 and 3e-20 (the 0.5/rho of
     the force block far outside the usual range).  These take the division's general path
     where the kernels otherwise share one reciprocal of rho between u = j/rho and 0.5/rho.
     Every code is compared with the oracle's."""
@@ -256,7 +257,7 @@ def test_codes_of_every_magnitude(oracle, q, collision, storage, ext, grid):
         quarter = np.uint16(0x7000 if storage == "FP16S" else 0x6800)
         smallest = np.uint16(1)
         fluid = np.flatnonzero((flags & 3) != 1)
-        plant = {"zero": fluid[::97], "tiny": fluid[31::97], "negative": fluid[61::97]}
+        plant = {"zero": fluid[::97], "tiny": fluid[31::97], "negative": fluid[61::97], "ones": fluid[79::97]}
         for kind, cells in plant.items():
             for n in cells:
                 codes[at[:, n]] = 0
@@ -268,8 +269,12 @@ def test_codes_of_every_magnitude(oracle, q, collision, storage, ext, grid):
                         F[[n, N + n, 2 * N + n]] = np.abs(F[[n, N + n, 2 * N + n]])
                 elif kind == "tiny":
                     codes[at[1, n]] = smallest
-                else:  # rho = -2^-8
+                elif kind == "negative":  # rho = -2^-8
                     codes[at[3, n]] = np.uint16(0xD800 if storage == "FP16S" else 0xB800)
+                else:  # rho = 2 - 2^-23 (all-ones mantissa): 1 - 2^-11, 2^-11 - 2^-22, 2^-23
+                    codes[at[0, n]] = 0
+                    for i, c in zip((1, 3, 5), (0x77FF, 0x4BFF, 0x1C00) if storage == "FP16S" else (0x77FE, 0x1FFE, 0x0004)):
+                        codes[at[i, n]] = np.uint16(c)
         lbm.write_ddfs(codes)
         ref.fi[:] = codes
         if F is not None:
@@ -279,7 +284,7 @@ def test_codes_of_every_magnitude(oracle, q, collision, storage, ext, grid):
         pops = oracle.decode(storage, codes)[at.ravel()].reshape(q, N).astype(np.float64)
         dens = 1.0 + pops.sum(axis=0)
         assert (dens[plant["zero"]] == 0).all() and (dens[plant["tiny"]] > 0).all() and (dens[plant["tiny"]] < 1e-7).all()
-        assert (dens[plant["negative"]] == -2.0 ** -8).all()
+        assert (dens[plant["negative"]] == -2.0 ** -8).all() and (dens[plant["ones"]] == 2.0 - 2.0 ** -23).all()
         assert dens[fluid].min() < -1 and dens[fluid].max() > 2
         lbm.run(1)
         ref.step(1)
