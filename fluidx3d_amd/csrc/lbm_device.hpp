@@ -135,6 +135,14 @@ template<> struct Storage<ST_FP16C> {
 		const uint32_t bits = ((uint32_t)(int32_t)(int16_t)h << 12) & 0x87FFF000u;
 		return __uint_as_float(bits) * 0x1p112f;
 	}
+	// decode of the low half of a two-cell word: one shift that takes the half
+	// sign-extended as its operand (SDWA) instead of a shift up and an arithmetic shift down
+	__device__ static __forceinline__ float decode_low(uint32_t w) {
+		uint32_t t;
+		asm("v_lshlrev_b32_sdwa %0, %1, sext(%2) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0"
+		    : "=v"(t) : "v"(12u), "v"(w));
+		return __uint_as_float(t & 0x87FFF000u) * 0x1p112f;
+	}
 	// With b = bits(x) + 0x800: the normal code's exponent field is (e_b - 112) mod 16 =
 	// e_b mod 16, so the normal magnitude is bits 12..26 of b.  The reference's denormal
 	// code ((0x7FF800 + m_b) >> (124 - e_b) + 1) >> 1 equals floor(|x|*2^25 + 1/2) for every
@@ -1091,7 +1099,10 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 	float f[Q];
 	if(liveA) {
 #pragma unroll
-		for(int i = 0; i < Q; i++) f[i] = S::decode((uint16_t)P[i]);
+		for(int i = 0; i < Q; i++) {
+			if constexpr(ST == ST_FP16C) f[i] = S::decode_low(P[i]);
+			else f[i] = S::decode((uint16_t)P[i]);
+		}
 		collide_cell<Q, TRT, EXT, false, shared_division(ST, EXT)>(a, n, flA & TYPE_BO, f, nullptr);
 #pragma unroll
 		for(int i = 0; i < Q; i++) P[i] = (P[i] & 0xFFFF0000u) | S::encode_word(f[i]);
