@@ -229,9 +229,10 @@ def test_codes_of_every_magnitude(oracle, q, collision, storage, ext, grid):
     beyond the formats' range.  Planted cells load populations that give rho = 0 with j != 0
     (u = +-inf before the clamp), rho = the smallest code (2^-39 FP16S, 2^-25 FP16C),
     rho = -2^-8 and rho = 2 - 2^-23 (an all-ones mantissa, the divisor for which a
-    reciprocal one ulp off would misround the quotient); TYPE_E cells carry preset densities 1e-30, 1e30 and 3e-20 (the 0.5/rho of
-    the force block far outside the usual range).  These take the division's general path
-    where the kernels otherwise share one reciprocal of rho between u = j/rho and 0.5/rho.
+    reciprocal one ulp off would misround the quotient); TYPE_E cells carry preset
+    densities 1e-30, 1e30 and 3e-20 (the 0.5/rho of the force block far outside the usual
+    range).  Zero, non-finite and extreme densities take the division's general path where
+    the kernels otherwise share one reciprocal of rho between u = j/rho and 0.5/rho.
     Every code is compared with the oracle's."""
     Nx, Ny, Nz = GRIDS[grid]
     N = Nx * Ny * Nz
