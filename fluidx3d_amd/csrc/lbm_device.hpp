@@ -354,6 +354,13 @@ struct SharedDivisor {
 };
 // The kernels that may use SharedDivisor for u = j/rho (see there).
 constexpr bool shared_division(uint32_t ST, uint32_t EXT) { return ST != ST_FP32 && !(EXT & EXT_MOVING_BOUNDARIES); }
+// D3Q19 TRT kernels whose whole-array collision forms each opposite pair's equilibrium
+// difference once (9 VALU instructions fewer per cell): those through buffer instructions
+// and the all-extension ones, which drop 10 VGPRs with it (5 -> 6 waves/SIMD; 256^3 FP16S
+// +8 %, FP16C +6.5 %).  Others held one VGPR more with it, which at 64 VGPRs costs a wave
+// per SIMD; in D3Q27 the all-extension kernel measured 2 % slower and the C3 sphere kernel
+// unchanged, so D3Q27 keeps the reference's form.
+constexpr bool shared_pair_difference(int Q, uint32_t EXT, bool BUF) { return Q == 19 && (BUF || (EXT & EXT_SUBGRID)); }
 
 // Force block of stream_collide/update_fields (src/kernel.cpp:1549-1563, 1847-1859)
 // plus the Brinkman porosity term: u = (u + f*0.5/rho)/(1+alpha/2), F = f - alpha*rho*u.
@@ -761,8 +768,11 @@ __device__ __forceinline__ void collide_pairs(float w, float wm, bool vf, bool i
 		const float wp = w;
 		const float c_taup = __builtin_fmaf(wp, -0.25f, 0.5f), c_taum = __builtin_fmaf(wm, -0.25f, 0.5f);
 		const float hwp = 0.5f * wp, hwm = 0.5f * wm;
-		auto trt = [&](float fe, float feb, float fh, float fhb, float F) {
-			return __builtin_fmaf(hwp, ((fe - fh) + feb) - fhb, __builtin_fmaf(hwm, ((fe - feb) - fh) + fhb, fh + F));
+		// dm = fe - feb.  D3Q19: the pair's two populations take dm and -dm, exactly the
+		// reference's feq[i]-feb[i] for both except that an exact zero difference becomes -0
+		// for one of them, which reaches the result only as the sign of an exact zero.
+		auto trt = [&](float fe, float feb, float fh, float fhb, float dm, float F) {
+			return __builtin_fmaf(hwp, ((fe - fh) + feb) - fhb, __builtin_fmaf(hwm, (dm - fh) + fhb, fh + F));
 		};
 		{
 			float F0 = 0.0f;
@@ -770,7 +780,7 @@ __device__ __forceinline__ void collide_pairs(float w, float wm, bool vf, bool i
 				const float g = fo.at(0);
 				F0 = __builtin_fmaf(c_taup, g + g, c_taum * (g - g));
 			}
-			const float c = trt(eq.f0, eq.f0, fhn[0], fhn[0], F0);
+			const float c = trt(eq.f0, eq.f0, fhn[0], fhn[0], eq.f0 - eq.f0, F0);
 			fhn[0] = isE ? eq.f0 : c;
 		}
 #pragma unroll
@@ -784,15 +794,18 @@ __device__ __forceinline__ void collide_pairs(float w, float wm, bool vf, bool i
 				Fa = __builtin_fmaf(c_taup, ga + gb, c_taum * (ga - gb));
 				Fb = __builtin_fmaf(c_taup, gb + ga, c_taum * (gb - ga));
 			}
-			const float ca = trt(ea, eb, fhn[k], fhn[k + 1], Fa);
-			const float cb = trt(eb, ea, fhn[k + 1], fhn[k], Fb);
+			const float d = ea - eb;
+			const float ca = trt(ea, eb, fhn[k], fhn[k + 1], d, Fa);
+			const float cb = trt(eb, ea, fhn[k + 1], fhn[k], Q == 19 ? -d : eb - ea, Fb);
 			fhn[k] = isE ? ea : ca;
 			fhn[k + 1] = isE ? eb : cb;
 		}
 	}
 }
 // SD: u = j/rho and 0.5/rho through SharedDivisor; only where shared_division allows it.
-template<int Q, bool TRT, uint32_t EXT, bool RECORD, bool SD = false>
+// SHARE: the whole-array TRT collision forms each opposite pair's equilibrium difference
+// once (shared_pair_difference).
+template<int Q, bool TRT, uint32_t EXT, bool RECORD, bool SD = false, bool SHARE = false>
 __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint32_t bo, float* fhn, float* moments) {
 	const bool isE = (a.ext & EXT_EQUILIBRIUM_BOUNDARIES) && bo == TYPE_E;
 	float rhon, uxn, uyn, uzn, rhom1;
@@ -878,8 +891,11 @@ __device__ __forceinline__ void collide_cell(const LbmArgs& a, uint64_t n, uint3
 #pragma unroll
 		for(int i = 0; i < Q; i++) {
 			const int b = i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); // opposite slot
+			// feq[i]-feq[b] of an even slot as the negated difference of its odd partner (the
+			// same value, but -0 for +0: only the sign of an exact zero in the result)
+			const float dm = !SHARE || (i & 1) || i == 0 ? feq[i] - feq[b] : -(feq[b] - feq[i]);
 			const float c = __builtin_fmaf(hwp, ((feq[i] - fhn[i]) + feq[b]) - fhn[b],
-			                               __builtin_fmaf(hwm, ((feq[i] - feq[b]) - fhn[i]) + fhn[b], forced ? fhn[i] + Fin[i] : fhn[i]));
+			                               __builtin_fmaf(hwm, (dm - fhn[i]) + fhn[b], forced ? fhn[i] + Fin[i] : fhn[i]));
 			out[i] = isE ? feq[i] : c;
 		}
 #pragma unroll
@@ -976,7 +992,7 @@ __device__ __forceinline__ void stream_collide_cell(LbmArgs a, float* moments) {
 		load_f<Q, ST, U, LIN, O32, BUF>(fi, a.N, s, a.odd, f);
 	}
 	if((a.ext & EXT_MOVING_BOUNDARIES) && bo == TYPE_MS) moving_boundaries<Q>(a, s, f);
-	collide_cell<Q, TRT, EXT, RECORD, shared_division(ST, EXT)>(a, n, bo, f, moments);
+	collide_cell<Q, TRT, EXT, RECORD, shared_division(ST, EXT), shared_pair_difference(Q, EXT, BUF)>(a, n, bo, f, moments);
 	store_f<Q, ST, U, LIN, O32, BUF>(fi, a.N, s, a.odd, f);
 }
 // XCD: the kernel may run on the 1-D XCD-mapped grid (a.xchunks > 0); false compiles the
@@ -1103,7 +1119,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 			if constexpr(ST == ST_FP16C) f[i] = S::decode_low(P[i]);
 			else f[i] = S::decode((uint16_t)P[i]);
 		}
-		collide_cell<Q, TRT, EXT, false, shared_division(ST, EXT)>(a, n, flA & TYPE_BO, f, nullptr);
+		collide_cell<Q, TRT, EXT, false, shared_division(ST, EXT), shared_pair_difference(Q, EXT, BUF)>(a, n, flA & TYPE_BO, f, nullptr);
 #pragma unroll
 		for(int i = 0; i < Q; i++) P[i] = (P[i] & 0xFFFF0000u) | S::encode_word(f[i]);
 	} else {
@@ -1114,7 +1130,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream_collide_pair(LbmArgs a) {
 	if(liveB) {
 #pragma unroll
 		for(int i = 0; i < Q; i++) f[i] = S::decode((uint16_t)(P[i] >> 16));
-		collide_cell<Q, TRT, EXT, false, shared_division(ST, EXT)>(a, n + 1u, flB & TYPE_BO, f, nullptr);
+		collide_cell<Q, TRT, EXT, false, shared_division(ST, EXT), shared_pair_difference(Q, EXT, BUF)>(a, n + 1u, flB & TYPE_BO, f, nullptr);
 #pragma unroll
 		for(int i = 0; i < Q; i++) P[i] = S::encode_high(f[i], P[i]); // (code << 16) | (P & 0xFFFF)
 	} else {
