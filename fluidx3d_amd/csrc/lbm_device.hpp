@@ -792,8 +792,8 @@ __device__ __forceinline__ void collide_pairs(float w, float wm, bool vf, bool i
 				float ga, gb;
 				fo.pair(k, ga, gb);
 				Fa = __builtin_fmaf(c_taup, ga + gb, c_taum * (ga - gb));
-				// D3Q19: c_taum*(gb-ga) as -(c_taum*(ga-gb)), equal but for the sign of a zero
-				Fb = __builtin_fmaf(c_taup, gb + ga, Q == 19 ? -(c_taum * (ga - gb)) : c_taum * (gb - ga));
+				// c_taum*(gb-ga) as -(c_taum*(ga-gb)), equal but for the sign of a zero
+				Fb = __builtin_fmaf(c_taup, gb + ga, -(c_taum * (ga - gb)));
 			}
 			const float d = ea - eb;
 			const float ca = trt(ea, eb, fhn[k], fhn[k + 1], d, Fa);
